@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the S3TTMc hot path on one MI355X.
+
+Workload (default): plain FP64 S3TTMc / HOOI on an order-4 symmetric tensor,
+n = 10,000, 1,000,000 unique sorted 4-tuples with Zipf(s=1.1) node
+popularity, values U(0,1), rank 10 (packed Y width C(12,3) = 220), seed 0.
+
+A step is one S3TTMc pass (sp_s3ttmc_dev) over the tensor with the factor
+and tree resident in HBM.  W untimed warm-up steps, then K timed steps
+between two stream synchronisations.  The JSON line also reports:
+  - roofline of the dominant kernel, timed live with HIP events on the
+    library's stream (algorithmic FP64 flops of the reference's tree
+    schedule / kernel time, against the FP64 peak of 78.6 TFLOP/s);
+  - HOOI time-to-fit in the FP64, FP32-mixed and BF16-mixed forms, each with
+    its fit error relative to the FP64 run;
+  - cpu_baseline: the reference's own construction + kronecker_product_opt5
+    (oracle/_ref, compiled from the reference sources) on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import symprop_amd as sp  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) peak
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def binom(n, k):
+    from math import comb
+    return comb(n, k)
+
+
+def f_alg(t, N, R, nnz):
+    """Tree-shared FP64 flops of one S3TTMc under the reference schedule:
+    sum_{l=1}^{N-2} L_l (l+1) 2 P_{l+1}  +  N nnz 2 P_{N-1}."""
+    P = lambda k: binom(R + k - 1, k)  # noqa: E731
+    f = 0
+    for l in range(1, N - 1):
+        f += t.level_size(l) * (l + 1) * 2 * P(l + 1)
+    f += N * nnz * 2 * P(N - 1)
+    return float(f)
+
+
+def workload(name):
+    if name == "zipf4":
+        return dict(kind=sp.GEN_ZIPF, order=4, dim=10000, nnz=1_000_000, param=1.1,
+                    communities=1, rank=10, max_iter=10, delta=0.5,
+                    desc="order-4 n=10000 1M unique Zipf(1.1) tuples U(0,1) values, rank 10")
+    if name == "planted6":
+        return dict(kind=sp.GEN_PLANTED, order=6, dim=50000, nnz=5_000_000, param=0.8,
+                    communities=8, rank=8, max_iter=20, delta=0.5,
+                    desc="order-6 n=50000 5M unique planted-8-community tuples, rank 8")
+    if name == "uniform5":
+        return dict(kind=sp.GEN_UNIFORM, order=5, dim=327, nnz=8000, param=0.0,
+                    communities=1, rank=8, max_iter=20, delta=0.1,
+                    desc="order-5 n=327 8000 uniform tuples, rank 8")
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(inds, vals, order, dim, R, budget_s=25.0):
+    """Reference construction + opt5 on the host cores (oracle/_ref)."""
+    try:
+        sys.path.insert(0, ROOT)
+        from oracle import ref
+    except Exception as e:  # pragma: no cover
+        return {"error": "oracle unavailable: %s" % e}
+    if not ref.available():
+        return {"error": "oracle/_ref not built"}
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    threads = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    rng = np.random.default_rng(0)
+    U = rng.standard_normal((dim, R))
+    P = binom(R + order - 2, order - 1)
+
+    def run(n):
+        a = np.ascontiguousarray(inds[:n])
+        v = np.ascontiguousarray(vals[:n])
+        t0 = time.perf_counter()
+        h = ref.lib().ref_build(order, dim, n, a.ctypes.data_as(ref.u32p),
+                                v.ctypes.data_as(ref.f64p))
+        tb = time.perf_counter() - t0
+        Y = np.zeros((dim, P))
+        t0 = time.perf_counter()
+        ref.lib().ref_opt5(h, U.ctypes.data_as(ref.f64p), R, Y.ctypes.data_as(ref.f64p),
+                           P, threads)
+        ts = time.perf_counter() - t0
+        ref.lib().ref_free(h)
+        return tb, ts
+
+    # bounded sample: a trial on a prefix sizes the measured prefix so that
+    # construction + one opt5 call stay within the budget
+    n = min(len(vals), 100_000)
+    tb, ts = run(n)
+    if n < len(vals):
+        scale = budget_s / max(tb + ts, 1e-3)
+        n = min(len(vals), int(n * scale))
+        tb, ts = run(n)
+    t_build, t_s3 = tb, ts
+    return {"value": n / t_s3, "unit": "nnz/s", "cores": threads, "kind": "reference",
+            "sample": "first %d of %d nonzeros; reference DTree+CompressedDTree build "
+                      "%.3f s, one kronecker_product_opt5 %.3f s with %d OpenMP threads"
+                      % (n, len(vals), t_build, t_s3, threads),
+            "insertion_s": t_build, "s3ttmc_s": t_s3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="zipf4")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hooi", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    w = workload(args.workload)
+    N, n, R = w["order"], w["dim"], w["rank"]
+    log("generating", w["desc"])
+    inds, vals = sp.generate(w["kind"], N, n, w["nnz"], w["param"], w["communities"], 0)
+    nnz = len(vals)
+    log("nnz", nnz)
+
+    t0 = time.perf_counter()
+    t = sp.SpSymtensor(n, N, inds, vals)
+    t.sync()
+    insertion_s = time.perf_counter() - t0
+    log("construction %.3f s" % insertion_s)
+
+    rng = np.random.default_rng(0)
+    U = np.linalg.qr(rng.standard_normal((n, R)))[0]
+    t.set_factor(U)
+    for _ in range(args.warmup):
+        t.s3ttmc_dev()
+    t.sync()
+    t.timing(True)
+    t.timing_reset()
+    t.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t.s3ttmc_dev()
+    t.sync()
+    wall = time.perf_counter() - t0
+    t.timing(False)
+    ms_step = wall * 1e3 / args.steps
+    k_ms, k_cnt = t.timing_get("s3ttmc_generic")
+    r_ms, r_cnt = t.timing_get("reduce_split_rows")
+    k_avg = k_ms / max(1, k_cnt)
+    Y = t.y()
+    if not np.all(np.isfinite(Y)):
+        raise SystemExit("non-finite S3TTMc output")
+    F = f_alg(t, N, R, nnz)
+    achieved = F / (k_avg * 1e-3) / 1e12
+    log("S3TTMc %.3f ms/step (kernel %.3f ms), %.3e nnz/s, %.2f TFLOP/s alg"
+        % (ms_step, k_avg, nnz / (ms_step * 1e-3), achieved))
+
+    out = {
+        "metric": "S3TTMc nnz/s + HOOI time-to-fit (mixed/learned vs FP64 GPU) at 1e-6 rel err",
+        "value": nnz / (ms_step * 1e-3),
+        "unit": "nnz/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, %s)" % w["desc"],
+        "config": {"workload": args.workload, "order": N, "dim": n, "nnz": nnz,
+                   "rank": R, "packed_width": binom(R + N - 2, N - 1),
+                   "construction_s": insertion_s},
+        "roofline": {"bound": "mfma", "roof": "FP64 arithmetic (MI355X FP64 vector = "
+                     "matrix peak)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "s3ttmc_generic",
+                     "kernel_ms": k_avg, "alg_flops": F,
+                     "split_reduce_ms": r_ms / max(1, r_cnt)},
+    }
+
+    if not args.no_hooi:
+        runs = {}
+        for name, prec in (("fp64", sp.PREC_FP64), ("fp32_mixed", sp.PREC_FP32),
+                           ("bf16_mixed", sp.PREC_BF16)):
+            try:
+                r = t.tucker(alg=0, rank=R, max_iter=w["max_iter"], rel_tol=1e-6,
+                             delta=w["delta"], seed=0, precision=prec)
+            except sp.SpError as e:
+                runs[name] = {"error": str(e)}
+                continue
+            runs[name] = {"seconds": r["seconds"], "iters": r["iters"],
+                          "fp64_sweeps": r["fp64_sweeps"],
+                          "final_error": float(r["errors"][-1])}
+            log("HOOI %s: %.3f s, %d iters, error %.12e" % (name, r["seconds"], r["iters"],
+                                                           r["errors"][-1]))
+        if "final_error" in runs.get("fp64", {}):
+            e64 = runs["fp64"]["final_error"]
+            for k, v in runs.items():
+                if "final_error" in v:
+                    v["rel_fit_error_vs_fp64"] = abs(v["final_error"] - e64) / e64
+        out["hooi_time_to_fit"] = runs
+
+    if not args.no_cpu:
+        log("cpu baseline (reference build on host cores)")
+        out["cpu_baseline"] = cpu_baseline(inds, vals, N, n, R)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+/* C-ABI of the MI355X-native S3TTMc hot path for symmetric sparse Tucker.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary.  Every
+ * function returns SP_OK (0) or an SP_ERR_* status; sp_last_error() gives
+ * the message of the last failure on the calling thread.  Host buffers stay
+ * owned by the caller; a context owns its device memory and one HIP stream.
+ * One context is used from one host thread at a time; independent contexts
+ * may run concurrently.
+ *
+ * Conventions taken from the reference (paths relative to its checkout):
+ *   - tensor indices are 1-based, each nonzero's index tuple sorted
+ *     ascending (src/utils/loader.cpp:30), row-major nnz x order uint32;
+ *   - factor U is n x R row-major FP64, row i-1 belongs to index i
+ *     (src/compressed_dtree.cpp:426);
+ *   - packed rows hold the P = C(R+N-2, N-1) entries of a symmetric
+ *     order-(N-1) tensor in lexicographic order of sorted multi-indices
+ *     (include/detail/dense/symtensor.hpp:129-142).
+ */
+#ifndef SYMPROP_AMD_H
+#define SYMPROP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_OK 0
+#define SP_ERR_ARG 1         /* invalid argument (null, size, unsorted tuple) */
+#define SP_ERR_UNSUPPORTED 2 /* order / rank / dimension outside the limits   */
+#define SP_ERR_DEVICE 3      /* HIP runtime or library failure                */
+#define SP_ERR_OOM 4         /* device allocation failed                      */
+#define SP_ERR_STATE 5       /* call out of order (no tree, no factor, ...)   */
+#define SP_ERR_NUMERIC 6     /* NaN / non-finite result detected              */
+#define SP_ERR_IO 7          /* file cannot be opened or parsed               */
+
+#define SP_PREC_FP64 0 /* FP64 everywhere                                    */
+#define SP_PREC_FP32 1 /* FP32 values, U rows and accumulation, FP64 Y out   */
+#define SP_PREC_BF16 2 /* BF16 operands, FP32 accumulation, FP64 Y out       */
+
+typedef struct sp_ctx sp_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *sp_last_error(void);
+
+/* Library version string. */
+const char *sp_version(void);
+
+/* Uploads the tensor and computes its norm.  Replaces the state set up by
+ * SpSymtensor::SpSymtensor (src/sparse/spsymtensor.cpp:16-28, declared at
+ * include/detail/sparse/spsymtensor.hpp:28-29); the tree is built by
+ * sp_cdtree_build.  Limits: 3 <= order <= 16, nnz >= 1, 1 <= index <= dim. */
+int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz,
+                  const uint32_t *inds, const double *vals, sp_ctx **out);
+int sp_ctx_destroy(sp_ctx *ctx);
+
+/* Shape of the tensor held by a context. */
+int sp_ctx_info(sp_ctx *ctx, uint32_t *order, uint32_t *dim, uint64_t *nnz);
+
+/* ||X|| = sqrt(sum v^2 N!/prod f!)  (SpSymtensor::calc_norm,
+ * src/sparse/spsymtensor.cpp:114-138). */
+int sp_norm(sp_ctx *ctx, double *out);
+
+/* Builds the compressed dimension tree on the GPU: replaces
+ * DTree::banerjee_insert (src/dtree.cpp:56-80) + CompressedDTree's
+ * constructor and init_dep_list (src/compressed_dtree.cpp:13-131). */
+int sp_cdtree_build(sp_ctx *ctx);
+
+/* Tree shape: depth = order-1 levels; level l holds the sorted distinct
+ * sub-multisets of size l+1. */
+int sp_cdtree_depth(sp_ctx *ctx, uint32_t *depth);
+int sp_cdtree_level_size(sp_ctx *ctx, uint32_t level, uint64_t *size);
+int sp_cdtree_num_owners(sp_ctx *ctx, uint64_t *count);
+
+/* Copies one level of the tree to host buffers (any may be null):
+ *   keys      [size x (level+1)]  full sorted tuples of the nodes
+ *   level_idx [size]              last index of each node (level_idx)
+ *   level_ptr [size+1]            CSR to level+1, or into the owners at the
+ *                                 leaf level (level_ptr)
+ *   dep_list  [size x (level+1)]  exclude-one parent indices (dep_list),
+ *                                 level >= 1 only
+ * matching the members of CompressedDTree
+ * (include/detail/compressed_dtree.hpp:52-59) bit for bit. */
+int sp_cdtree_export(sp_ctx *ctx, uint32_t level, uint32_t *keys,
+                     uint32_t *level_idx, uint64_t *level_ptr,
+                     uint64_t *dep_list);
+/* Owner list (CompressedDTree::nonzero): (rem index, value) in leaf order. */
+int sp_cdtree_export_owners(sp_ctx *ctx, uint32_t *rem, double *val);
+
+/* Uploads / downloads the factor U (n x R).  R may change between calls. */
+int sp_factor_upload(sp_ctx *ctx, const double *U, uint32_t R);
+int sp_factor_download(sp_ctx *ctx, double *U);
+
+/* Packed S3TTMc of the device factor into the device Y (n x P), overwriting
+ * Y.  Replaces SpSymtensor::S3TTMc(res, U, R, partial_sym_res=true)
+ * (include/detail/sparse/spsymtensor.hpp:38-39, src/sparse/spsymtensor.cpp:
+ * 30-98) and CompressedDTree::kronecker_product_opt5
+ * (include/detail/compressed_dtree.hpp:21-22).  precision: SP_PREC_*. */
+int sp_s3ttmc_dev(sp_ctx *ctx, int precision);
+
+/* Host-buffer convenience form of the same call: uploads U, runs, and copies
+ * the packed Y into Y (n rows, leading dimension ldy >= P). */
+int sp_s3ttmc(sp_ctx *ctx, const double *U, uint32_t R, double *Y,
+              uint64_t ldy, int precision);
+
+/* Copies the device Y (n x P) to the host with leading dimension ldy. */
+int sp_y_download(sp_ctx *ctx, double *Y, uint64_t ldy);
+/* Packed width P for the current R (C(R+N-2, N-1)). */
+int sp_packed_width(sp_ctx *ctx, uint64_t *P);
+
+/* core = U^T Y (R x P) on the device; copied to core (if non-null).
+ * Replaces matrix::gemmtn (src/dense/matrixops.cpp:45-56) as called at
+ * src/sparse/spsymtensor.cpp:270 and 321. */
+int sp_gemmtn(sp_ctx *ctx, double *core);
+
+/* U = Y diag(W) core^T (n x R) on the device, W = symtens_coeff of order
+ * N-1.  Replaces matrix::symgemm (src/dense/matrixops.cpp:102-124) as
+ * called at src/sparse/spsymtensor.cpp:271. */
+int sp_symgemm(sp_ctx *ctx);
+
+/* G = Y^T diag(W) Y (P x P) on the device; copied to G (if non-null).  The
+ * n x n Gram Y diag(W) Y^T of matrix::symsyrk (src/dense/matrixops.cpp:
+ * 126-146) shares its nonzero spectrum; this is its P x P form. */
+int sp_gram(sp_ctx *ctx, double *G);
+
+/* U = top-R left singular vectors of the unpacked Y (n x R^(N-1)), computed
+ * from the packed Y through its weighted Gram.  Replaces the
+ * matrix::leftsvd(Y) of src/sparse/spsymtensor.cpp:225 and the
+ * symsyrk + syevd pair of src/sparse/spsymtensor.cpp:318-319.
+ * sigma (if non-null) receives the R singular values, descending. */
+int sp_leftsvd_y(sp_ctx *ctx, double *sigma);
+
+/* Thin QR of the device U in place (matrix::qr, src/dense/matrixops.cpp:
+ * 58-75, called at src/sparse/spsymtensor.cpp:272). */
+int sp_qr(sp_ctx *ctx);
+
+/* Row clipping to norm <= delta, then orthonormalisation of U (the left
+ * singular vectors of the clipped U): regularize,
+ * src/sparse/spsymtensor.cpp:168-199. */
+int sp_regularize(sp_ctx *ctx, double delta);
+
+/* sqrt(sum_r sum_p core[r,p]^2 W[p]) of the device core
+ * (symtens_norm, src/dense/symtensor.cpp:97-106). */
+int sp_core_norm(sp_ctx *ctx, double *out);
+
+/* One decomposition run of the host drop-in layer on a context:
+ * alg 0 = SpSymtensor::TuckerHOOI, 1 = TuckerHOQRI, 2 = TuckerAltHOOI
+ * (include/detail/sparse/spsymtensor.hpp:40-42, src/sparse/spsymtensor.cpp:
+ * 201-337) with the fields of Config (include/detail/tucker.hpp:14-23).
+ * init: "rnd" or "hoevd"; U0 (dim x rank, optional) overrides it.
+ * precision: SP_PREC_* of the S3TTMc iterations; reduced forms switch to
+ * FP64 once |error change| < switch_tol (<= 0: 1e-3 for BF16, 1e-5 for
+ * FP32) and then iterate in FP64 until |error change| < rel_tol.
+ * Outputs: U_out (dim x rank), errors (max_iter entries, may be null), the
+ * iteration count, the count of FP64 iterations after the switch and the
+ * wall time of the loop. */
+int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
+                  double rel_tol, double delta, uint64_t seed, const char *init,
+                  int precision, double switch_tol, const double *U0,
+                  int verbose, double *U_out, double *errors, uint64_t *iters,
+                  uint64_t *fp64_sweeps, double *seconds);
+
+/* Kernel timing with HIP events on the context's stream.  While enabled,
+ * every kernel launch of the library is bracketed by events; sp_timing_get
+ * sums the durations of launches whose name matches `name` exactly. */
+int sp_timing_enable(sp_ctx *ctx, int on);
+int sp_timing_get(sp_ctx *ctx, const char *name, double *total_ms,
+                  uint64_t *launches);
+int sp_timing_reset(sp_ctx *ctx);
+
+/* Synchronises the context's stream. */
+int sp_sync(sp_ctx *ctx);
+
+/* Input side.  .tns reader with the reference's semantics
+ * (SymtensLoader, src/utils/loader.cpp:8-36): header "order dim", then per
+ * line `order` indices and a value; each tuple is sorted; reading stops at
+ * the first failed read.  Call with inds/vals null to get order, dim and
+ * nnz, then again with buffers of that size. */
+int sp_load_tns(const char *path, uint32_t *order, uint32_t *dim,
+                uint64_t *nnz, uint32_t *inds, double *vals);
+
+/* Seeded synthetic tensors (unique sorted tuples, lexicographically sorted):
+ *   kind 0: uniform indices over [1,dim], values 1.0
+ *   kind 1: bounded Zipf(s) node popularity over [1,dim], values U(0,1)
+ *   kind 2: planted communities (param = p_in, communities = k), values 1.0
+ * Draws tuples until nnz unique tuples exist (kind 0 stops after nnz draws
+ * and keeps the unique ones).  Returns the count in *nnz_out. */
+int sp_generate(int kind, uint32_t order, uint32_t dim, uint64_t nnz,
+                double param, uint32_t communities, uint64_t seed,
+                uint32_t *inds, double *vals, uint64_t *nnz_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYMPROP_AMD_H */

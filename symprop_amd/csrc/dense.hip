@@ -1,0 +1,447 @@
+// Dense Y-side operations of the Tucker loops, FP64 on the device.
+//
+//   gemmtn   core = U^T Y                     (src/dense/matrixops.cpp:45-56)
+//   symgemm  U = Y diag(W) core^T             (src/dense/matrixops.cpp:102-124)
+//   gram     G = diag(sqrt W) Y^T Y diag(sqrt W), the P x P Gram whose
+//            nonzero spectrum equals that of symsyrk's n x n Y diag(W) Y^T
+//            (src/dense/matrixops.cpp:126-146)
+//   leftsvd  top-R left singular vectors of the unpacked Y: with
+//            Z = Y diag(sqrt W), Z Z^T = Y_unpacked Y_unpacked^T, so
+//            U = Z V_R Lambda_R^{-1/2} from the top-R eigenpairs of G
+//   qr       thin Q of U (rocSOLVER geqrf/orgqr)  (src/dense/matrixops.cpp:58-75)
+//   regularize  row clip to delta + left singular vectors of U
+//            (src/sparse/spsymtensor.cpp:168-199)
+// Reductions over the n rows go through fixed-order partial sums, so results
+// are deterministic.
+#include "sp_internal.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace spamd {
+
+namespace {
+
+constexpr int RB = 16;  // rank columns per register block
+
+// partial[b][r][p] = sum over rows of block b of A[i][r] * B[i][p]
+__global__ void __launch_bounds__(256)
+atb_partial(const double *__restrict__ A, u32 R, const double *__restrict__ B,
+            u64 P, u64 n, u64 rows_per, double *__restrict__ partial) {
+  const u64 p = (u64)blockIdx.y * 256 + threadIdx.x;
+  const u64 i0 = (u64)blockIdx.x * rows_per;
+  const u64 i1 = min(n, i0 + rows_per);
+  for (u32 r0 = 0; r0 < R; r0 += RB) {
+    double acc[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) acc[q] = 0.0;
+    if (p < P) {
+      for (u64 i = i0; i < i1; ++i) {
+        const double y = B[i * P + p];
+        const double *a = A + i * R + r0;
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+          if (r0 + q < R) acc[q] += a[q] * y;
+      }
+#pragma unroll
+      for (int q = 0; q < RB; ++q)
+        if (r0 + q < R) partial[((u64)blockIdx.x * R + r0 + q) * P + p] = acc[q];
+    }
+  }
+}
+
+__global__ void sum_partials(const double *__restrict__ partial, u64 nb, u64 len,
+                             double *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= len) return;
+  double s = 0.0;
+  for (u64 b = 0; b < nb; ++b) s += partial[b * len + t];
+  out[t] = s;
+}
+
+// out[i][r] = sum_p Y[i][p] * M[r][p]; one wave per row.
+__global__ void __launch_bounds__(256)
+y_times_mt(const double *__restrict__ Y, u64 n, u64 P, const double *__restrict__ M,
+           u32 R, double *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const u64 i = (u64)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  for (u32 r0 = 0; r0 < R; r0 += RB) {
+    double acc[RB];
+#pragma unroll
+    for (int q = 0; q < RB; ++q) acc[q] = 0.0;
+    for (u64 p = lane; p < P; p += 64) {
+      const double y = Y[i * P + p];
+#pragma unroll
+      for (int q = 0; q < RB; ++q)
+        if (r0 + q < R) acc[q] += y * M[(u64)(r0 + q) * P + p];
+    }
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      double v = acc[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0 && r0 + q < R) out[i * R + r0 + q] = v;
+    }
+  }
+}
+
+__global__ void scale_cols(const double *__restrict__ core, const double *__restrict__ W,
+                           u64 R, u64 P, double *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R * P) out[t] = core[t] * W[t % P];
+}
+
+// Gram tile kernel: partial[kb][tile] for 64x64 tiles (tp <= tq) of Y^T Y.
+constexpr int GT = 64;
+constexpr int GK = 16;
+__global__ void __launch_bounds__(256)
+gram_partial(const double *__restrict__ Y, u64 n, u64 P, u64 rows_per,
+             const int2 *__restrict__ tiles, double *__restrict__ partial) {
+  __shared__ double sa[GK][GT + 1];
+  __shared__ double sb[GK][GT + 1];
+  const int2 tl = tiles[blockIdx.x];
+  const u64 p0 = (u64)tl.x * GT, q0 = (u64)tl.y * GT;
+  const u64 i0 = (u64)blockIdx.y * rows_per;
+  const u64 i1 = min(n, i0 + rows_per);
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double acc[4][4] = {};
+  for (u64 k = i0; k < i1; k += GK) {
+    for (int t = threadIdx.x; t < GK * GT; t += 256) {
+      const int kk = t / GT, c = t % GT;
+      const u64 i = k + kk;
+      sa[kk][c] = (i < i1 && p0 + c < P) ? Y[i * P + p0 + c] : 0.0;
+      sb[kk][c] = (i < i1 && q0 + c < P) ? Y[i * P + q0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = sa[kk][ty + 16 * u];
+        b[u] = sb[kk][tx + 16 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+    __syncthreads();
+  }
+  double *out = partial + ((u64)blockIdx.y * gridDim.x + blockIdx.x) * GT * GT;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[(ty + 16 * u) * GT + tx + 16 * v] = acc[u][v];
+}
+
+__global__ void gram_finish(const double *__restrict__ partial, u64 nk, u64 ntiles,
+                            const int2 *__restrict__ tiles, u64 P,
+                            const double *__restrict__ W, double *__restrict__ G) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles * GT * GT) return;
+  const u64 tile = t / (GT * GT);
+  const int u = (int)((t % (GT * GT)) / GT), v = (int)(t % GT);
+  const int2 tl = tiles[tile];
+  const u64 p = (u64)tl.x * GT + u, q = (u64)tl.y * GT + v;
+  if (p >= P || q >= P) return;
+  double s = 0.0;
+  for (u64 k = 0; k < nk; ++k) s += partial[(k * ntiles + tile) * GT * GT + u * GT + v];
+  s *= sqrt(W[p]) * sqrt(W[q]);
+  G[p * P + q] = s;
+  G[q * P + p] = s;
+}
+
+// M[r][p] = sqrt(W[p]) * V[:, P-1-r][p] / sqrt(lambda_{P-1-r})
+__global__ void svd_map(const double *__restrict__ V, const double *__restrict__ lam,
+                        const double *__restrict__ W, u64 P, u32 R, double floor_,
+                        double *__restrict__ M) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (u64)R * P) return;
+  const u64 r = t / P, p = t % P;
+  const u64 j = P - 1 - r;
+  const double l = lam[j] > floor_ ? lam[j] : floor_;
+  M[t] = sqrt(W[p]) * V[j * P + p] / sqrt(l);
+}
+
+__global__ void transpose_nr(const double *__restrict__ in, u64 rows, u64 cols,
+                             double *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * cols) return;
+  const u64 i = t / cols, j = t % cols;
+  out[j * rows + i] = in[t];
+}
+
+__global__ void clip_rows(double *__restrict__ U, u64 n, u32 R, double delta) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (u32 j = 0; j < R; ++j) s += U[i * R + j] * U[i * R + j];
+  const double nrm = sqrt(s);
+  if (delta < nrm) {
+    const double mul = delta / nrm;
+    for (u32 j = 0; j < R; ++j) U[i * R + j] *= mul;
+  }
+}
+
+// Host: cyclic Jacobi eigen-decomposition of a small symmetric matrix
+// (row-major n x n).  Eigenvalues descending in w, eigenvectors in the
+// columns of V (row-major).
+void jacobi_eig(std::vector<double> A, int n, std::vector<double> &w,
+                std::vector<double> &V) {
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double a = A[(size_t)i * n + j];
+        tot += a * a;
+        if (i != j) off += a * a;
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) /
+                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = c * vkp - s * vkq;
+          V[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return A[(size_t)a * n + a] > A[(size_t)b * n + b];
+  });
+  w.resize(n);
+  std::vector<double> V2((size_t)n * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = A[(size_t)ord[j] * n + ord[j]];
+    for (int i = 0; i < n; ++i) V2[(size_t)i * n + j] = V[(size_t)i * n + ord[j]];
+  }
+  V.swap(V2);
+}
+
+rocblas_handle handle_for(Ctx &c) {
+  static thread_local rocblas_handle h = nullptr;
+  if (!h) {
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      throw Error(SP_ERR_DEVICE, "rocblas_create_handle failed");
+  }
+  rocblas_set_stream(h, c.stream);
+  return h;
+}
+
+// out = A^T B over n rows (A: n x R, B: n x P) -> R x P, deterministic.
+void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *out,
+         const char *name) {
+  const u64 ctiles = (P + 255) / 256;
+  u64 nb = std::max<u64>(1, std::min<u64>(512, (n + 127) / 128));
+  const u64 rows_per = (n + nb - 1) / nb;
+  nb = (n + rows_per - 1) / rows_per;
+  c.d_scratch.reserve(nb * R * P * sizeof(double));
+  {
+    ScopedTimer t(c, name);
+    atb_partial<<<dim3((unsigned)nb, (unsigned)ctiles), 256, 0, c.stream>>>(
+        A, R, B, P, n, rows_per, c.d_scratch.as<double>());
+    SP_CHECK_LAUNCH();
+    sum_partials<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
+        c.d_scratch.as<double>(), nb, (u64)R * P, out);
+    SP_CHECK_LAUNCH();
+  }
+}
+
+// U <- U V Lambda^{-1/2} from the R x R Gram U^T U: the left singular
+// vectors of U in descending order (first pass), or U <- U (U^T U)^{-1/2}
+// (symmetric pass, which keeps the column order of a nearly orthonormal U).
+void orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
+  const u32 R = c.R;
+  const u64 n = c.dim;
+  DevBuf g((u64)R * R * sizeof(double));
+  atb(c, c.d_U.as<double>(), R, c.d_U.as<double>(), R, n, g.as<double>(), "gram_u");
+  std::vector<double> G((size_t)R * R);
+  SP_HIP(hipMemcpyAsync(G.data(), g.p, G.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> w, V;
+  jacobi_eig(G, (int)R, w, V);
+  // M[r][s] (R x R, "r" = output column) = V[s][r] / sqrt(w[r])
+  std::vector<double> M((size_t)R * R);
+  const double floor_ = std::max(w[0], 1e-300) * 1e-30;
+  for (u32 r = 0; r < R; ++r)
+    for (u32 s = 0; s < R; ++s) {
+      if (!symmetric) {
+        M[(size_t)r * R + s] = V[(size_t)s * R + r] / std::sqrt(std::max(w[r], floor_));
+      } else {
+        double v = 0.0;
+        for (u32 q = 0; q < R; ++q)
+          v += V[(size_t)s * R + q] * V[(size_t)r * R + q] / std::sqrt(std::max(w[q], floor_));
+        M[(size_t)r * R + s] = v;
+      }
+    }
+  SP_HIP(hipMemcpyAsync(g.p, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice,
+                        c.stream));
+  tmp.reserve(n * R * sizeof(double));
+  y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
+                                                   g.as<double>(), R, tmp.as<double>());
+  SP_CHECK_LAUNCH();
+  SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+void dense_gemmtn(Ctx &c) {
+  c.d_core.reserve((u64)c.R * c.P * sizeof(double));
+  atb(c, c.d_U.as<double>(), c.R, c.d_Y.as<double>(), c.P, c.dim,
+      c.d_core.as<double>(), "gemmtn");
+}
+
+void dense_symgemm(Ctx &c) {
+  const u64 n = c.dim;
+  const u32 R = c.R;
+  DevBuf M((u64)R * c.P * sizeof(double));
+  DevBuf out(n * R * sizeof(double));
+  ScopedTimer t(c, "symgemm");
+  scale_cols<<<grid_for((u64)R * c.P, 256), 256, 0, c.stream>>>(
+      c.d_core.as<double>(), c.d_W.as<double>(), R, c.P, M.as<double>());
+  SP_CHECK_LAUNCH();
+  y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, c.P,
+                                                   M.as<double>(), R, out.as<double>());
+  SP_CHECK_LAUNCH();
+  SP_HIP(hipMemcpyAsync(c.d_U.p, out.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+void dense_gram(Ctx &c) {
+  const u64 P = c.P, n = c.dim;
+  const u64 nt = (P + GT - 1) / GT;
+  std::vector<int2> tiles;
+  for (u64 a = 0; a < nt; ++a)
+    for (u64 b = a; b < nt; ++b) tiles.push_back(make_int2((int)a, (int)b));
+  const u64 ntiles = tiles.size();
+  u64 nk = std::max<u64>(1, std::min<u64>((n + 255) / 256, 1024 / ntiles + 1));
+  const u64 rows_per = (n + nk - 1) / nk;
+  nk = (n + rows_per - 1) / rows_per;
+  DevBuf dt(ntiles * sizeof(int2));
+  SP_HIP(hipMemcpyAsync(dt.p, tiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice,
+                        c.stream));
+  c.d_scratch.reserve(nk * ntiles * GT * GT * sizeof(double));
+  c.d_G.reserve(P * P * sizeof(double));
+  {
+    ScopedTimer t(c, "gram");
+    gram_partial<<<dim3((unsigned)ntiles, (unsigned)nk), 256, 0, c.stream>>>(
+        c.d_Y.as<double>(), n, P, rows_per, dt.as<int2>(), c.d_scratch.as<double>());
+    SP_CHECK_LAUNCH();
+    gram_finish<<<grid_for(ntiles * GT * GT, 256), 256, 0, c.stream>>>(
+        c.d_scratch.as<double>(), nk, ntiles, dt.as<int2>(), P, c.d_W.as<double>(),
+        c.d_G.as<double>());
+    SP_CHECK_LAUNCH();
+  }
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+void dense_leftsvd_y(Ctx &c, double *sigma) {
+  dense_gram(c);
+  const u64 P = c.P, n = c.dim;
+  const u32 R = c.R;
+  if (R > P) throw Error(SP_ERR_ARG, "rank exceeds packed width");
+  DevBuf D(P * sizeof(double)), E(P * sizeof(double)), info(sizeof(rocblas_int));
+  rocblas_handle h = handle_for(c);
+  {
+    ScopedTimer t(c, "syevd");
+    rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
+                                         (rocblas_int)P, c.d_G.as<double>(),
+                                         (rocblas_int)P, D.as<double>(), E.as<double>(),
+                                         info.as<rocblas_int>());
+    if (st != rocblas_status_success) throw Error(SP_ERR_DEVICE, "rocsolver_dsyevd failed");
+  }
+  std::vector<double> lam(P);
+  SP_HIP(hipMemcpyAsync(lam.data(), D.p, P * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  const double lmax = std::max(std::fabs(lam[P - 1]), 1e-300);
+  if (sigma)
+    for (u32 r = 0; r < R; ++r) sigma[r] = std::sqrt(std::max(lam[P - 1 - r], 0.0));
+  DevBuf M((u64)R * P * sizeof(double));
+  svd_map<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
+      c.d_G.as<double>(), D.as<double>(), c.d_W.as<double>(), P, R, lmax * 1e-30,
+      M.as<double>());
+  SP_CHECK_LAUNCH();
+  {
+    ScopedTimer t(c, "y_times_v");
+    y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, P,
+                                                     M.as<double>(), R, c.d_U.as<double>());
+    SP_CHECK_LAUNCH();
+  }
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+void dense_qr(Ctx &c) {
+  const u64 n = c.dim;
+  const u32 R = c.R;
+  DevBuf At(n * R * sizeof(double)), tau(R * sizeof(double));
+  transpose_nr<<<grid_for(n * R, 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
+                                                           At.as<double>());
+  SP_CHECK_LAUNCH();
+  rocblas_handle h = handle_for(c);
+  ScopedTimer t(c, "qr");
+  if (rocsolver_dgeqrf(h, (rocblas_int)n, (rocblas_int)R, At.as<double>(), (rocblas_int)n,
+                       tau.as<double>()) != rocblas_status_success)
+    throw Error(SP_ERR_DEVICE, "rocsolver_dgeqrf failed");
+  if (rocsolver_dorgqr(h, (rocblas_int)n, (rocblas_int)R, (rocblas_int)R, At.as<double>(),
+                       (rocblas_int)n, tau.as<double>()) != rocblas_status_success)
+    throw Error(SP_ERR_DEVICE, "rocsolver_dorgqr failed");
+  transpose_nr<<<grid_for(n * R, 256), 256, 0, c.stream>>>(At.as<double>(), R, n,
+                                                           c.d_U.as<double>());
+  SP_CHECK_LAUNCH();
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+void dense_regularize(Ctx &c, double delta) {
+  const u64 n = c.dim;
+  clip_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n, c.R, delta);
+  SP_CHECK_LAUNCH();
+  DevBuf tmp;
+  orthonormalize_gram(c, tmp, false);
+  orthonormalize_gram(c, tmp, true);
+}
+
+double dense_core_norm(Ctx &c) {
+  std::vector<double> core((u64)c.R * c.P), W(c.P);
+  SP_HIP(hipMemcpyAsync(core.data(), c.d_core.p, core.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipMemcpyAsync(W.data(), c.d_W.p, W.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  double s = 0.0;
+  for (u32 r = 0; r < c.R; ++r)
+    for (u64 p = 0; p < c.P; ++p) s += std::pow(core[(u64)r * c.P + p], 2) * W[p];
+  return std::sqrt(s);
+}
+
+}  // namespace spamd

@@ -1,0 +1,374 @@
+// C-ABI entry points (declared in include/symprop_amd.h).
+#include "sp_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+
+using namespace spamd;
+
+struct sp_ctx {
+  Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(sp_ctx *ctx, F &&f) {
+  try {
+    f();
+    return SP_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    if (ctx) ctx->c.last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return SP_ERR_OOM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return SP_ERR_DEVICE;
+  }
+}
+
+void need(bool cond, int code, const char *msg) {
+  if (!cond) throw Error(code, msg);
+}
+
+void need_factor(Ctx &c) {
+  need(c.R > 0 && c.d_U.p, SP_ERR_STATE, "no factor uploaded");
+}
+
+void need_y(Ctx &c) {
+  need(c.d_Y.p && c.P > 0, SP_ERR_STATE, "no S3TTMc result on the device");
+}
+
+}  // namespace
+
+namespace spamd {
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace spamd
+
+extern "C" {
+
+const char *sp_last_error(void) { return g_last_error.c_str(); }
+
+const char *sp_version(void) { return "symprop_amd 0.1 (gfx950)"; }
+
+int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *inds,
+                  const double *vals, sp_ctx **out) {
+  return guard(nullptr, [&] {
+    need(out != nullptr, SP_ERR_ARG, "out is null");
+    *out = nullptr;
+    need(inds && vals, SP_ERR_ARG, "null tensor buffers");
+    need(order >= 3 && order <= 16, SP_ERR_UNSUPPORTED, "order must be in [3,16]");
+    need(dim >= 1, SP_ERR_ARG, "dim must be >= 1");
+    need(nnz >= 1, SP_ERR_ARG, "nnz must be >= 1");
+    for (uint64_t i = 0; i < nnz; ++i) {
+      const uint32_t *t = inds + i * order;
+      for (uint32_t j = 0; j < order; ++j) {
+        need(t[j] >= 1 && t[j] <= dim, SP_ERR_ARG, "index outside [1,dim]");
+        need(j == 0 || t[j - 1] <= t[j], SP_ERR_ARG, "index tuple not sorted");
+      }
+    }
+    int ndev = 0;
+    SP_HIP(hipGetDeviceCount(&ndev));
+    need(ndev > 0, SP_ERR_DEVICE, "no HIP device");
+    auto *h = new sp_ctx();
+    Ctx &c = h->c;
+    try {
+      c.order = order;
+      c.dim = dim;
+      c.nnz = nnz;
+      c.norm = host_norm(order, nnz, inds, vals);
+      SP_HIP(hipGetDevice(&c.device));
+      SP_HIP(hipStreamCreate(&c.stream));
+      c.d_inds.alloc(nnz * order * sizeof(uint32_t));
+      c.d_vals.alloc(nnz * sizeof(double));
+      SP_HIP(hipMemcpy(c.d_inds.p, inds, nnz * order * sizeof(uint32_t),
+                       hipMemcpyHostToDevice));
+      SP_HIP(hipMemcpy(c.d_vals.p, vals, nnz * sizeof(double), hipMemcpyHostToDevice));
+    } catch (...) {
+      if (c.stream) (void)hipStreamDestroy(c.stream);
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int sp_ctx_destroy(sp_ctx *ctx) {
+  if (!ctx) return SP_OK;
+  return guard(nullptr, [&] {
+    Ctx &c = ctx->c;
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (auto &t : c.timers) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    hipStream_t s = c.stream;
+    delete ctx;
+    if (s) (void)hipStreamDestroy(s);
+  });
+}
+
+int sp_ctx_info(sp_ctx *ctx, uint32_t *order, uint32_t *dim, uint64_t *nnz) {
+  return guard(ctx, [&] {
+    need(ctx && order && dim && nnz, SP_ERR_ARG, "null argument");
+    *order = ctx->c.order;
+    *dim = ctx->c.dim;
+    *nnz = ctx->c.nnz;
+  });
+}
+
+int sp_norm(sp_ctx *ctx, double *out) {
+  return guard(ctx, [&] {
+    need(ctx && out, SP_ERR_ARG, "null argument");
+    *out = ctx->c.norm;
+  });
+}
+
+int sp_cdtree_build(sp_ctx *ctx) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    ScopedTimer t(ctx->c, "cdtree_build");
+    cdtree_build(ctx->c);
+  });
+}
+
+int sp_cdtree_depth(sp_ctx *ctx, uint32_t *depth) {
+  return guard(ctx, [&] {
+    need(ctx && depth, SP_ERR_ARG, "null argument");
+    need(ctx->c.built, SP_ERR_STATE, "tree not built");
+    *depth = (uint32_t)ctx->c.depth;
+  });
+}
+
+int sp_cdtree_level_size(sp_ctx *ctx, uint32_t level, uint64_t *size) {
+  return guard(ctx, [&] {
+    need(ctx && size, SP_ERR_ARG, "null argument");
+    need(ctx->c.built, SP_ERR_STATE, "tree not built");
+    need(level < (uint32_t)ctx->c.depth, SP_ERR_ARG, "level out of range");
+    *size = ctx->c.levels[level].size;
+  });
+}
+
+int sp_cdtree_num_owners(sp_ctx *ctx, uint64_t *count) {
+  return guard(ctx, [&] {
+    need(ctx && count, SP_ERR_ARG, "null argument");
+    need(ctx->c.built, SP_ERR_STATE, "tree not built");
+    *count = ctx->c.n_owners;
+  });
+}
+
+int sp_cdtree_export(sp_ctx *ctx, uint32_t level, uint32_t *keys, uint32_t *level_idx,
+                     uint64_t *level_ptr, uint64_t *dep_list) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    need(ctx->c.built, SP_ERR_STATE, "tree not built");
+    need(level < (uint32_t)ctx->c.depth, SP_ERR_ARG, "level out of range");
+    cdtree_export_level(ctx->c, (int)level, keys, level_idx, level_ptr, dep_list);
+  });
+}
+
+int sp_cdtree_export_owners(sp_ctx *ctx, uint32_t *rem, double *val) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    need(ctx->c.built, SP_ERR_STATE, "tree not built");
+    cdtree_export_owners(ctx->c, rem, val);
+  });
+}
+
+int sp_factor_upload(sp_ctx *ctx, const double *U, uint32_t R) {
+  return guard(ctx, [&] {
+    need(ctx && U, SP_ERR_ARG, "null argument");
+    need(R >= 1 && R <= 255, SP_ERR_UNSUPPORTED, "rank must be in [1,255]");
+    Ctx &c = ctx->c;
+    if (c.R != R) {
+      c.R = R;
+      c.P = symsize(R, c.order - 1);
+      c.chunk_R = 0;
+    }
+    c.d_U.reserve((u64)c.dim * R * sizeof(double));
+    SP_HIP(hipMemcpyAsync(c.d_U.p, U, (u64)c.dim * R * sizeof(double),
+                          hipMemcpyHostToDevice, c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int sp_factor_download(sp_ctx *ctx, double *U) {
+  return guard(ctx, [&] {
+    need(ctx && U, SP_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    need_factor(c);
+    SP_HIP(hipMemcpyAsync(U, c.d_U.p, (u64)c.dim * c.R * sizeof(double),
+                          hipMemcpyDeviceToHost, c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int sp_s3ttmc_dev(sp_ctx *ctx, int precision) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    need(c.built, SP_ERR_STATE, "tree not built");
+    need_factor(c);
+    need(precision >= SP_PREC_FP64 && precision <= SP_PREC_BF16, SP_ERR_ARG,
+         "unknown precision");
+    s3ttmc_run(c, precision);
+  });
+}
+
+int sp_s3ttmc(sp_ctx *ctx, const double *U, uint32_t R, double *Y, uint64_t ldy,
+              int precision) {
+  int rc = sp_factor_upload(ctx, U, R);
+  if (rc) return rc;
+  rc = sp_s3ttmc_dev(ctx, precision);
+  if (rc) return rc;
+  return sp_y_download(ctx, Y, ldy);
+}
+
+int sp_y_download(sp_ctx *ctx, double *Y, uint64_t ldy) {
+  return guard(ctx, [&] {
+    need(ctx && Y, SP_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    need_y(c);
+    need(ldy >= c.P, SP_ERR_ARG, "ldy smaller than the packed width");
+    SP_HIP(hipMemcpy2DAsync(Y, ldy * sizeof(double), c.d_Y.p, c.P * sizeof(double),
+                            c.P * sizeof(double), c.dim, hipMemcpyDeviceToHost,
+                            c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int sp_packed_width(sp_ctx *ctx, uint64_t *P) {
+  return guard(ctx, [&] {
+    need(ctx && P, SP_ERR_ARG, "null argument");
+    need_factor(ctx->c);
+    *P = ctx->c.P;
+  });
+}
+
+int sp_gemmtn(sp_ctx *ctx, double *core) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    need_factor(c);
+    need_y(c);
+    dense_gemmtn(c);
+    if (core) {
+      SP_HIP(hipMemcpyAsync(core, c.d_core.p, (u64)c.R * c.P * sizeof(double),
+                            hipMemcpyDeviceToHost, c.stream));
+    }
+    SP_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int sp_symgemm(sp_ctx *ctx) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    need_factor(c);
+    need_y(c);
+    need(c.d_core.p, SP_ERR_STATE, "no core (call sp_gemmtn first)");
+    dense_symgemm(c);
+  });
+}
+
+int sp_gram(sp_ctx *ctx, double *G) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    need_y(c);
+    dense_gram(c);
+    if (G) {
+      SP_HIP(hipMemcpyAsync(G, c.d_G.p, c.P * c.P * sizeof(double),
+                            hipMemcpyDeviceToHost, c.stream));
+      SP_HIP(hipStreamSynchronize(c.stream));
+    }
+  });
+}
+
+int sp_leftsvd_y(sp_ctx *ctx, double *sigma) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    need_factor(c);
+    need_y(c);
+    dense_leftsvd_y(c, sigma);
+  });
+}
+
+int sp_qr(sp_ctx *ctx) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    need_factor(ctx->c);
+    need(ctx->c.R <= ctx->c.dim, SP_ERR_ARG, "rank exceeds dimension");
+    dense_qr(ctx->c);
+  });
+}
+
+int sp_regularize(sp_ctx *ctx, double delta) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    need_factor(ctx->c);
+    dense_regularize(ctx->c, delta);
+  });
+}
+
+int sp_core_norm(sp_ctx *ctx, double *out) {
+  return guard(ctx, [&] {
+    need(ctx && out, SP_ERR_ARG, "null argument");
+    need(ctx->c.d_core.p, SP_ERR_STATE, "no core");
+    *out = dense_core_norm(ctx->c);
+  });
+}
+
+int sp_timing_enable(sp_ctx *ctx, int on) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    ctx->c.profile = on != 0;
+  });
+}
+
+int sp_timing_get(sp_ctx *ctx, const char *name, double *total_ms, uint64_t *launches) {
+  return guard(ctx, [&] {
+    need(ctx && name && total_ms && launches, SP_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    SP_HIP(hipStreamSynchronize(c.stream));
+    double tot = 0.0;
+    uint64_t cnt = 0;
+    for (auto &t : c.timers) {
+      if (t.name != name) continue;
+      float ms = 0.f;
+      SP_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+      tot += ms;
+      ++cnt;
+    }
+    *total_ms = tot;
+    *launches = cnt;
+  });
+}
+
+int sp_timing_reset(sp_ctx *ctx) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    SP_HIP(hipStreamSynchronize(c.stream));
+    for (auto &t : c.timers) {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    c.timers.clear();
+  });
+}
+
+int sp_sync(sp_ctx *ctx) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    SP_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// Internal declarations of the MI355X S3TTMc library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/symprop_amd.h"
+
+namespace spamd {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+using i64 = int64_t;
+using u128 = unsigned __int128;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define SP_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      throw ::spamd::Error(e_ == hipErrorOutOfMemory ? SP_ERR_OOM           \
+                                                     : SP_ERR_DEVICE,       \
+                           std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SP_CHECK_LAUNCH() SP_HIP(hipGetLastError())
+
+// Owning device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) return;
+    SP_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  // Grows (never shrinks) to at least b bytes; contents are not kept.
+  void reserve(size_t b) { if (b > bytes) alloc(b); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Packed-layout tables for one (R, k): lexicographic enumeration of sorted
+// k-tuples over [0,R) (symtensor.hpp:129-142) and, for the polynomial
+// product p_k = p_{k-1} * (u . x), the gather terms of every output entry.
+struct PackedTables {
+  u32 R = 0;
+  int k = 0;
+  u64 P = 0;                      // C(R+k-1, k)
+  std::vector<u32> term_ptr;      // P+1
+  std::vector<u32> term;          // (src index << 8) | j
+  std::vector<double> fact_prod;  // prod_j f_j!   (k-tuple multiplicities)
+  std::vector<double> weight;     // k!/prod f_j!  (symtens_coeff)
+};
+PackedTables make_tables(u32 R, int k);
+u64 symsize(u64 R, u64 k);
+u64 binom(u64 n, u64 k);
+
+// Device-resident compressed dimension tree.
+struct DevLevel {
+  u64 size = 0;
+  DevBuf keys;       // u64 or u128 packed tuples (key_words x 8 bytes each)
+  DevBuf level_idx;  // u32 [size]
+  DevBuf level_ptr;  // u64 [size+1]
+  DevBuf dep_list;   // u64 [size*(l+1)], l >= 1
+};
+
+struct Chunk {
+  u64 o_begin, o_end;  // owner range (rem-sorted order)
+  u32 r_begin, r_end;  // Y rows written by this chunk
+  i64 partial;         // >= 0: piece of one row, result goes to partial slot
+};
+
+struct SplitRow {
+  u32 row;
+  u32 slot_begin, slot_end;
+};
+
+struct Timer {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  u32 order = 0, dim = 0;
+  u64 nnz = 0;
+  int depth = 0;       // order-1 levels
+  int bits = 0;        // bits per index in packed keys
+  int key_words = 0;   // 1 (u64) or 2 (u128) for the widest level
+  double norm = 0.0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  DevBuf d_inds;  // u32 nnz*order (1-based, sorted rows)
+  DevBuf d_vals;  // f64 nnz
+
+  bool built = false;
+  std::vector<DevLevel> levels;
+  DevBuf own_rem;   // u32 [owners], leaf order
+  DevBuf own_val;   // f64 [owners], leaf order
+  u64 n_owners = 0;
+
+  // S3TTMc work list (owners sorted by rem, then leaf)
+  DevBuf w_tuple;   // u32 [owners * m], 0-based rows of the leaf
+  DevBuf w_coef;    // f64 [owners], v * multinomial / N!
+  DevBuf w_coef32;  // f32 [owners]
+  DevBuf w_segptr;  // u64 [dim+1]
+  std::vector<u64> h_segptr;
+
+  // factor / outputs
+  u32 R = 0;
+  u64 P = 0;        // packed Y width C(R+N-2, N-1)
+  DevBuf d_U;       // f64 n x R
+  DevBuf d_U32;     // f32 n x R
+  DevBuf d_Ubf;     // bf16 n x R
+  DevBuf d_Y;       // f64 n x P
+  DevBuf d_Y32;     // f32 n x P
+  DevBuf d_core;    // f64 R x P
+  DevBuf d_W;       // f64 P  (symtens_coeff of order N-1)
+  DevBuf d_G;       // f64 P x P
+  DevBuf d_scratch;
+  DevBuf d_partial;
+
+  // chunking of the work list for the current R
+  std::vector<Chunk> h_chunks;
+  std::vector<SplitRow> h_split;
+  DevBuf d_chunks, d_split;
+  u64 n_partial = 0;
+  u32 chunk_R = 0;
+
+  // tables per level for the current R
+  std::vector<PackedTables> tabs;  // index k = 1..m
+  DevBuf d_tab_ptr, d_tab_term, d_fact;
+  std::vector<u64> tab_ptr_off, tab_term_off;
+
+  // profiling
+  bool profile = false;
+  std::vector<Timer> timers;
+
+  std::string last_error;
+};
+
+// construction (cdtree.hip)
+void cdtree_build(Ctx &c);
+void cdtree_export_level(Ctx &c, int l, u32 *keys, u32 *level_idx,
+                         u64 *level_ptr, u64 *dep_list);
+void cdtree_export_owners(Ctx &c, u32 *rem, double *val);
+
+// sort / scan primitives (sort.hip)
+void exclusive_scan_u64(const u64 *in, u64 *out, u64 n, hipStream_t s);
+template <class K>
+void radix_sort(K *keys, K *keys_tmp, u32 *vals, u32 *vals_tmp, u64 n,
+                int bits, hipStream_t s, K **keys_out, u32 **vals_out);
+
+// S3TTMc (s3ttmc.hip)
+void s3ttmc_prepare(Ctx &c, u32 R);
+void s3ttmc_run(Ctx &c, int precision);
+
+// dense Y-side ops (dense.hip)
+void dense_gemmtn(Ctx &c);
+void dense_symgemm(Ctx &c);
+void dense_gram(Ctx &c);
+void dense_leftsvd_y(Ctx &c, double *sigma);
+void dense_qr(Ctx &c);
+void dense_regularize(Ctx &c, double delta);
+double dense_core_norm(Ctx &c);
+
+// error string of the C-ABI (sp_api.hip)
+void set_last_error(const std::string &msg);
+
+// input side (input.cpp)
+double host_norm(u32 order, u64 nnz, const u32 *inds, const double *vals);
+
+// timing helpers
+struct ScopedTimer {
+  Ctx &c;
+  Timer t;
+  bool on;
+  ScopedTimer(Ctx &cx, const char *name) : c(cx), on(cx.profile) {
+    if (!on) return;
+    t.name = name;
+    SP_HIP(hipEventCreate(&t.a));
+    SP_HIP(hipEventCreate(&t.b));
+    SP_HIP(hipEventRecord(t.a, c.stream));
+  }
+  ~ScopedTimer() {
+    if (!on) return;
+    (void)hipEventRecord(t.b, c.stream);
+    c.timers.push_back(t);
+  }
+};
+
+inline unsigned grid_for(u64 n, unsigned block) {
+  u64 g = (n + block - 1) / block;
+  return (unsigned)(g == 0 ? 1 : g);
+}
+
+}  // namespace spamd

@@ -1,0 +1,489 @@
+// Host C++ implementation of the drop-in decomposition API (see symtucker.hpp).
+#include "symtucker.hpp"
+
+#include "../../include/symprop_amd.h"
+#include "sp_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+#include <mutex>
+#include <random>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace symprop {
+
+namespace {
+
+void ck(int rc) {
+  if (rc != SP_OK)
+    throw std::runtime_error(std::string("symprop_amd: ") + sp_last_error());
+}
+
+index_t binom(index_t n, index_t k) {
+  if (k > n) return 0;
+  if (k * 2 > n) k = n - k;
+  if (k == 0) return 1;
+  index_t r = n;
+  for (index_t i = 2; i <= k; ++i) {
+    r *= (n - i + 1);
+    r /= i;
+  }
+  return r;
+}
+
+// spsymtensor.cpp:148-158
+real_t rel_residual(const std::vector<real_t> &x, const std::vector<real_t> &y) {
+  real_t residual = 0.0, norm = 0.0;
+  for (size_t i = 0; i < x.size(); i++) {
+    residual += std::pow((x[i] - y[i]), 2);
+    norm += std::pow(x[i], 2);
+  }
+  return residual / norm;
+}
+
+// symtensor.cpp:78-106
+real_t symtens_rel_res(const std::vector<real_t> &c1, const std::vector<real_t> &c2,
+                       const std::vector<real_t> &coeff, size_t nrows) {
+  real_t residual = 0.0, norm = 0.0;
+  const size_t P = coeff.size();
+  for (size_t i = 0; i < nrows; i++)
+    for (size_t j = 0; j < P; j++) {
+      residual += std::pow(c1[i * P + j] - c2[i * P + j], 2) * coeff[j];
+      norm += std::pow(c1[i * P + j], 2) * coeff[j];
+    }
+  return residual / norm;
+}
+
+real_t symtens_norm(const std::vector<real_t> &core, const std::vector<real_t> &coeff,
+                    size_t nrows) {
+  real_t norm = 0.0;
+  const size_t P = coeff.size();
+  for (size_t i = 0; i < nrows; i++)
+    for (size_t j = 0; j < P; j++) norm += std::pow(core[i * P + j], 2) * coeff[j];
+  return std::sqrt(norm);
+}
+
+real_t plain_norm(const std::vector<real_t> &x) {
+  real_t n = 0.0;
+  for (real_t v : x) n += std::pow(v, 2);
+  return std::sqrt(n);
+}
+
+real_t auto_switch_tol(const Config &c) {
+  if (c.switch_tol > 0) return c.switch_tol;
+  const real_t t = (c.precision == SP_PREC_BF16) ? 1e-3 : 1e-5;
+  return std::max(t, c.rel_tol);
+}
+
+// Precision of the next S3TTMc in the mixed forms: reduced until the error
+// change falls below the switch tolerance, FP64 afterwards.
+struct PrecisionSchedule {
+  int prec;
+  bool low;
+  real_t switch_tol;
+  index_t fp64 = 0;
+  explicit PrecisionSchedule(const Config &c)
+      : prec(c.precision), low(c.precision != SP_PREC_FP64),
+        switch_tol(auto_switch_tol(c)) {}
+  int current() const { return low ? prec : SP_PREC_FP64; }
+  // Returns true when the loop may stop (FP64 phase and converged).
+  bool step(real_t diff, real_t rel_tol) {
+    if (low) {
+      if (diff < switch_tol) low = false;
+      return false;
+    }
+    if (prec != SP_PREC_FP64) ++fp64;
+    return diff < rel_tol;
+  }
+};
+
+template <class F>
+void timed(const char *name, sp_ctx *ctx, F &&f) {
+  FunctionTimer t(name);
+  f();
+  ck(sp_sync(ctx));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ packed helpers
+index_t symtens_size(index_t dim, index_t order) { return binom(order + dim - 1, order); }
+
+std::vector<real_t> symtens_coeff(size_t dim, size_t order) {
+  std::vector<real_t> out;
+  std::vector<dim_t> idx(order, 0);
+  auto fact = [](int n) {
+    real_t f = 1;
+    for (int i = 2; i <= n; ++i) f *= i;
+    return f;
+  };
+  while (true) {
+    real_t d = 1;
+    int run = 1;
+    for (size_t q = 1; q <= order; ++q) {
+      if (q < order && idx[q] == idx[q - 1]) ++run;
+      else { d *= fact(run); run = 1; }
+    }
+    out.push_back(fact((int)order) / d);
+    int p = (int)order - 1;
+    while (p >= 0 && idx[p] == dim - 1) --p;
+    if (p < 0) break;
+    idx[p]++;
+    for (size_t q = p + 1; q < order; ++q) idx[q] = idx[p];
+  }
+  return out;
+}
+
+// symtensor.hpp:96-116: every distinct permutation of each packed entry
+void unpack_symtens(const real_t *packed, dim_t dim, order_t order, real_t *full) {
+  std::vector<dim_t> idx(order, 0);
+  index_t p = 0;
+  while (true) {
+    std::vector<dim_t> perm = idx;
+    do {
+      index_t lin = 0;
+      for (order_t j = 0; j < order; ++j) lin = lin * dim + perm[j];
+      full[lin] = packed[p];
+    } while (std::next_permutation(perm.begin(), perm.end()));
+    ++p;
+    int q = (int)order - 1;
+    while (q >= 0 && idx[q] == dim - 1) --q;
+    if (q < 0) break;
+    idx[q]++;
+    for (order_t r = q + 1; r < order; ++r) idx[r] = idx[q];
+  }
+}
+
+// ------------------------------------------------------------ FunctionTimer
+namespace {
+struct TimeAccumulator {
+  std::mutex m;
+  std::unordered_map<std::string, long long> total, calls;
+  static TimeAccumulator &get() {
+    static TimeAccumulator a;
+    return a;
+  }
+};
+}  // namespace
+
+FunctionTimer::FunctionTimer(std::string_view name)
+    : m_start(std::chrono::high_resolution_clock::now()), m_name(name) {}
+FunctionTimer::~FunctionTimer() { stop(); }
+
+void FunctionTimer::stop() {
+  if (m_stopped) return;
+  m_stopped = true;
+  const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                      std::chrono::high_resolution_clock::now() - m_start)
+                      .count();
+  auto &a = TimeAccumulator::get();
+  std::lock_guard<std::mutex> lk(a.m);
+  a.total[m_name] += us;
+  a.calls[m_name]++;
+}
+
+// function_timer.cpp:25-46 report format
+void FunctionTimer::printTimes() {
+  auto &a = TimeAccumulator::get();
+  std::lock_guard<std::mutex> lk(a.m);
+  std::vector<std::pair<std::string, long long>> v(a.total.begin(), a.total.end());
+  std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return y.second < x.second; });
+  std::cout << "Accumulated function times:\n";
+  for (const auto &e : v) {
+    std::streamsize ss = std::cout.precision();
+    const double ms = static_cast<double>(e.second) / 1000.0;
+    std::cout << std::fixed << std::setprecision(3) << std::setw(10) << ms << " ms  | "
+              << std::setw(10) << a.calls[e.first] << " calls | " << e.first << "\n";
+    std::cout.precision(ss);
+  }
+}
+
+void FunctionTimer::clear() {
+  auto &a = TimeAccumulator::get();
+  std::lock_guard<std::mutex> lk(a.m);
+  a.total.clear();
+  a.calls.clear();
+}
+
+// ------------------------------------------------------------ SymTuckerDecomp
+// tucker.cpp:25-36
+SymTuckerDecomp::SymTuckerDecomp(const Config &config, dim_t dim, order_t order)
+    : _dim(dim), _rank(config.rank), _order(order) {
+  _core.resize((size_t)std::pow(_rank, order));
+  _factor.resize((size_t)dim * _rank);
+  std::normal_distribution<real_t> distr(0.0, 1.0);
+  std::mt19937 engine{config.seed};
+  for (size_t i = 0; i < _factor.size(); i++) _factor[i] = distr(engine);
+}
+
+// tucker.cpp:40-45
+real_t SymTuckerDecomp::reconstruct_error(const SpSymtensor &target) const {
+  const real_t sp = target.norm();
+  const real_t cn = plain_norm(_core);
+  return std::sqrt(sp * sp - cn * cn) / sp;
+}
+
+// tucker.cpp:47-61, including its core indexing _core[i * _rank + j]
+void SymTuckerDecomp::print(std::ostream &os) const {
+  os << "Mode-1 matricized C:\n";
+  for (size_t i = 0; i < _rank; i++) {
+    for (size_t j = 0; j < _core.size() / _rank; j++) os << _core[i * _rank + j] << ",";
+    os << "\n";
+  }
+  os << "Factor U:\n";
+  for (size_t i = 0; i < _dim; i++) {
+    for (size_t j = 0; j < _rank; j++) os << _factor[i * _rank + j] << ",";
+    os << "\n";
+  }
+}
+
+// ------------------------------------------------------------ SpSymtensor
+SpSymtensor::SpSymtensor(dim_t dim, order_t order, const std::vector<dim_t> &unnz_inds,
+                         const std::vector<real_t> &unnzs)
+    : _dim(dim), _order(order), _unnz(unnzs.size()), _owns(true) {
+  START_FUNCTION_TIMER();
+  if (unnz_inds.size() != (size_t)_unnz * order)
+    throw std::invalid_argument("index array size does not match nnz * order");
+  ck(sp_ctx_create(order, dim, _unnz, unnz_inds.data(), unnzs.data(), &_ctx));
+  ck(sp_norm(_ctx, &_norm));
+  {
+    FunctionTimer t("CompressedDTree");
+    ck(sp_cdtree_build(_ctx));
+  }
+  std::cout << "Finished creating compressed dtree\n";
+}
+
+SpSymtensor::SpSymtensor(sp_ctx *ctx) : _ctx(ctx), _owns(false) {
+  ck(sp_ctx_info(_ctx, &_order, &_dim, &_unnz));
+  ck(sp_norm(_ctx, &_norm));
+}
+
+SpSymtensor::~SpSymtensor() {
+  if (_owns && _ctx) sp_ctx_destroy(_ctx);
+}
+
+// spsymtensor.cpp:30-110 (packed result; unpacked rows when !partial_sym_res)
+void SpSymtensor::S3TTMc(std::vector<real_t> &res, const std::vector<real_t> &U, dim_t R,
+                         bool partial_sym_res) const {
+  START_FUNCTION_TIMER();
+  const size_t P = symtens_size(R, _order - 1);
+  const size_t full = (size_t)std::pow(R, _order - 1);
+  const size_t ncols = partial_sym_res ? P : full;
+  if (res.size() != (size_t)_dim * ncols) res.resize((size_t)_dim * ncols);
+  if (U.size() != (size_t)_dim * R) throw std::invalid_argument("U must be dim x R");
+  ck(sp_factor_upload(_ctx, U.data(), R));
+  ck(sp_s3ttmc_dev(_ctx, SP_PREC_FP64));
+  ck(sp_y_download(_ctx, res.data(), ncols));
+  if (!partial_sym_res) {
+    std::vector<real_t> row(P);
+    for (size_t i = 0; i < _dim; i++) {
+      std::copy(res.begin() + i * ncols, res.begin() + i * ncols + P, row.begin());
+      unpack_symtens(row.data(), R, _order - 1, res.data() + i * ncols);
+    }
+  }
+}
+
+// spsymtensor.cpp:339-383
+void SpSymtensor::init_factor(std::vector<real_t> &U, const Config &c) const {
+  FunctionTimer t("hosvd_init");
+  if (c.init == "hosvd") {
+    printf("Error: sparse SVD requires MKL.\n");
+    return;
+  }
+  if (c.init == "hoevd") {
+    hoevd_init(U, c);
+    return;
+  }
+  if (c.init == "given") {
+    if (c.U0.size() != U.size()) throw std::invalid_argument("U0 must be dim x rank");
+    U = c.U0;
+  }
+}
+
+void SpSymtensor::hoevd_init(std::vector<real_t> &, const Config &) const {
+  throw std::runtime_error("hoevd initialisation is not available in this build");
+}
+
+// spsymtensor.cpp:201-252
+SymTuckerDecomp SpSymtensor::TuckerHOOI(const Config &c) const {
+  START_FUNCTION_TIMER();
+  const auto t0 = std::chrono::steady_clock::now();
+  SymTuckerDecomp decomp(c, _dim, _order);
+  auto &core = decomp._core;
+  auto &factor = decomp._factor;
+  auto core_prev = core;
+  init_factor(factor, c);
+  const dim_t R = c.rank;
+  const size_t P = symtens_size(R, _order - 1);
+  const size_t full = (size_t)std::pow(R, _order - 1);
+  std::vector<real_t> core_p((size_t)R * P);
+  ck(sp_factor_upload(_ctx, factor.data(), R));
+  PrecisionSchedule sched(c);
+  real_t prev_error = 0.0;
+  for (size_t iter = 0; iter < c.max_iter; iter++) {
+    timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
+    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, sched.current())); });
+    timed("leftsvd", _ctx, [&] { ck(sp_leftsvd_y(_ctx, nullptr)); });
+    timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
+    for (dim_t r = 0; r < R; ++r)
+      unpack_symtens(core_p.data() + (size_t)r * P, R, _order - 1, core.data() + r * full);
+    const real_t rel_res = rel_residual(core_prev, core);
+    const real_t error = decomp.reconstruct_error(*this);
+    if (c.verbose)
+      std::cout << std::scientific << std::setprecision(12) << "Iteration: " << iter
+                << " relative residual: " << rel_res << " error " << error << "\n";
+    decomp._errors.push_back(error);
+    const real_t error_diff = std::abs(prev_error - error);
+    if (sched.step(error_diff, c.rel_tol)) break;
+    prev_error = error;
+    core_prev = core;
+  }
+  ck(sp_factor_download(_ctx, factor.data()));
+  decomp._fp64_sweeps = sched.fp64;
+  decomp._seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return decomp;
+}
+
+// spsymtensor.cpp:254-299
+SymTuckerDecomp SpSymtensor::TuckerHOQRI(const Config &c) const {
+  START_FUNCTION_TIMER();
+  const auto t0 = std::chrono::steady_clock::now();
+  SymTuckerDecomp decomp(c, _dim, _order);
+  auto &factor = decomp._factor;
+  init_factor(factor, c);
+  const dim_t R = c.rank;
+  const size_t ncols = symtens_size(R, _order - 1);
+  std::vector<real_t> core((size_t)R * ncols), core_prev((size_t)R * ncols);
+  const std::vector<real_t> coeff = symtens_coeff(R, _order - 1);
+  ck(sp_factor_upload(_ctx, factor.data(), R));
+  PrecisionSchedule sched(c);
+  real_t prev_error = 0.0;
+  for (size_t iter = 0; iter < c.max_iter; iter++) {
+    timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
+    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, sched.current())); });
+    timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core.data())); });
+    timed("symgemm", _ctx, [&] { ck(sp_symgemm(_ctx)); });
+    timed("qr", _ctx, [&] { ck(sp_qr(_ctx)); });
+    const real_t rel_res = symtens_rel_res(core, core_prev, coeff, R);
+    const real_t core_norm = symtens_norm(core, coeff, R);
+    const real_t error = std::sqrt(_norm * _norm - core_norm * core_norm) / _norm;
+    if (c.verbose)
+      std::cout << std::scientific << std::setprecision(12) << "Iteration: " << iter
+                << " relative residual: " << rel_res << " error " << error << "\n";
+    decomp._errors.push_back(error);
+    core_prev = core;
+    const real_t error_diff = std::abs(prev_error - error);
+    if (sched.step(error_diff, c.rel_tol)) break;
+    prev_error = error;
+  }
+  const size_t unpacked = (size_t)std::pow(R, _order - 1);
+  for (dim_t i = 0; i < R; i++)
+    unpack_symtens(core.data() + (size_t)i * ncols, R, _order - 1,
+                   decomp._core.data() + (size_t)i * unpacked);
+  ck(sp_factor_download(_ctx, factor.data()));
+  decomp._fp64_sweeps = sched.fp64;
+  decomp._seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return decomp;
+}
+
+// spsymtensor.cpp:301-337.  The n x n symsyrk + syevd pair is replaced by the
+// P x P weighted Gram of the packed Y (same leading eigenvectors); columns
+// are returned in ascending eigenvalue order as matrixops.cpp:91-97 does.
+SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
+  START_FUNCTION_TIMER();
+  const auto t0 = std::chrono::steady_clock::now();
+  SymTuckerDecomp decomp(c, _dim, _order);
+  auto &core = decomp._core;
+  auto &factor = decomp._factor;
+  auto factor_prev = factor;
+  const std::vector<real_t> coeff = symtens_coeff(c.rank, _order - 1);
+  init_factor(factor, c);
+  const dim_t R = c.rank;
+  const size_t ncols = symtens_size(R, _order - 1);
+  std::vector<real_t> core_p((size_t)R * ncols);
+  ck(sp_factor_upload(_ctx, factor.data(), R));
+  PrecisionSchedule sched(c);
+  real_t prev_error = 0.0;
+  for (size_t iter = 0; iter < c.max_iter; iter++) {
+    timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
+    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, sched.current())); });
+    timed("syevd", _ctx, [&] {
+      ck(sp_leftsvd_y(_ctx, nullptr));
+      ck(sp_factor_download(_ctx, factor.data()));
+      for (size_t i = 0; i < _dim; ++i)
+        std::reverse(factor.begin() + i * R, factor.begin() + (i + 1) * R);
+      ck(sp_factor_upload(_ctx, factor.data(), R));
+    });
+    const real_t rel_res = rel_residual(factor, factor_prev);
+    timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
+    std::copy(core_p.begin(), core_p.end(), core.begin());
+    const real_t core_norm = symtens_norm(core_p, coeff, R);
+    const real_t error = std::sqrt(_norm * _norm - core_norm * core_norm) / _norm;
+    if (c.verbose)
+      std::cout << std::scientific << std::setprecision(10) << "Iteration: " << iter
+                << " relative residual: " << rel_res << " error " << error << "\n";
+    decomp._errors.push_back(error);
+    const real_t error_diff = std::abs(prev_error - error);
+    if (sched.step(error_diff, c.rel_tol)) break;
+    prev_error = error;
+    factor_prev = factor;
+  }
+  decomp._fp64_sweeps = sched.fp64;
+  decomp._seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return decomp;
+}
+
+}  // namespace symprop
+
+// ------------------------------------------------------------ C entry
+extern "C" int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
+                             double rel_tol, double delta, uint64_t seed, const char *init,
+                             int precision, double switch_tol, const double *U0,
+                             int verbose, double *U_out, double *errors,
+                             uint64_t *iters, uint64_t *fp64_sweeps, double *seconds) {
+  using namespace symprop;
+  try {
+    if (!ctx || !U_out || !iters) throw spamd::Error(SP_ERR_ARG, "null argument");
+    if (alg < 0 || alg > 2) throw spamd::Error(SP_ERR_ARG, "alg must be 0, 1 or 2");
+    if (rank < 1) throw spamd::Error(SP_ERR_ARG, "rank must be >= 1");
+    SpSymtensor T(ctx);
+    Config c;
+    c.rank = rank;
+    c.max_iter = max_iter;
+    c.rel_tol = rel_tol;
+    c.delta = delta;
+    c.seed = seed;
+    c.init = init ? init : "rnd";
+    c.precision = precision;
+    c.switch_tol = switch_tol;
+    c.verbose = verbose != 0;
+    if (U0) {
+      c.init = "given";
+      c.U0.assign(U0, U0 + (size_t)T.dim() * rank);
+    }
+    const SymTuckerDecomp r =
+        alg == 0 ? T.TuckerHOOI(c) : alg == 1 ? T.TuckerHOQRI(c) : T.TuckerAltHOOI(c);
+    std::copy(r.factor().begin(), r.factor().end(), U_out);
+    if (errors) std::copy(r.errors().begin(), r.errors().end(), errors);
+    *iters = r.iterations();
+    if (fp64_sweeps) *fp64_sweeps = r.fp64_sweeps();
+    if (seconds) *seconds = r.seconds();
+    return SP_OK;
+  } catch (const spamd::Error &e) {
+    spamd::set_last_error(e.what());
+    return e.code;
+  } catch (const std::invalid_argument &e) {
+    spamd::set_last_error(e.what());
+    return SP_ERR_ARG;
+  } catch (const std::exception &e) {
+    spamd::set_last_error(e.what());
+    return SP_ERR_DEVICE;
+  }
+}

@@ -1,0 +1,233 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle.
+
+Integer work (tree construction) must be bit-exact; FP64 S3TTMc must agree
+with the oracle's opt5 restatement to a W-weighted relative Frobenius error
+<= 1e-12 (different summation order); FP32 to <= 2e-5.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import symprop_amd as sp
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def werr(Y, Yref, R, order):
+    W = oracle.symtens_coeff(R, order - 1)
+    num = np.sqrt(((Y - Yref) ** 2 * W).sum())
+    den = np.sqrt((Yref ** 2 * W).sum())
+    return num / max(den, 1e-300)
+
+
+def check_tree(t, o, d, inds, vals):
+    T = oracle.Tree(o, d, inds, vals)
+    assert t.depth() == o - 1
+    for l in range(o - 1):
+        a, b = t.level(l), T.level(l)
+        assert t.level_size(l) == T.level_size(l), l
+        assert np.array_equal(a["keys"], b["keys"]), l
+        assert np.array_equal(a["level_idx"], b["level_idx"]), l
+        assert np.array_equal(a["level_ptr"], b["level_ptr"]), l
+        if l >= 1:
+            assert np.array_equal(a["dep_list"], b["dep_list"]), l
+    r1, v1 = t.owners()
+    r2, v2 = T.owners()
+    assert np.array_equal(r1, r2) and np.array_equal(v1, v2)
+    return T
+
+
+def test_construction_contact5(contact5):
+    o, d, inds, vals = contact5
+    say("construction contact-5")
+    t = sp.SpSymtensor(d, o, inds, vals)
+    check_tree(t, o, d, inds, vals)
+    with open(os.path.join(GOLDEN, "golden.json")) as f:
+        g = json.load(f)["contact5"]
+    assert [t.level_size(l) for l in range(4)] == [e["size"] for e in g["levels"]]
+    assert t.norm() == g["norm"]
+
+
+def test_construction_small_golden():
+    z = np.load(os.path.join(GOLDEN, "small_trees.npz"))
+    for ci in range(5):
+        tt, v = z["c%d_inds" % ci], z["c%d_vals" % ci]
+        N, n = tt.shape[1], int(tt.max())
+        say("construction golden case", ci, N, n)
+        t = sp.SpSymtensor(n, N, tt, v)
+        for l in range(N - 1):
+            lv = t.level(l)
+            assert np.array_equal(lv["level_idx"], z["c%d_l%d_idx" % (ci, l)])
+            assert np.array_equal(lv["level_ptr"], z["c%d_l%d_ptr" % (ci, l)])
+            if l >= 1:
+                assert np.array_equal(lv["dep_list"], z["c%d_l%d_dep" % (ci, l)])
+        rem, _ = t.owners()
+        assert np.array_equal(rem, z["c%d_rem" % ci])
+
+
+@pytest.mark.parametrize("N,n,cnt,seed", [
+    (3, 9, 60, 1), (4, 30, 400, 2), (5, 12, 300, 3), (6, 8, 200, 4),
+    (7, 6, 150, 5),
+    (5, 70000, 3000, 6),   # 17-bit indices: 128-bit leaf keys
+    (7, 100000, 2000, 7),  # 102-bit leaf keys
+    (3, 1, 1, 8),          # single index value
+])
+def test_construction_random(N, n, cnt, seed):
+    rng = np.random.default_rng(seed)
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(cnt, N)), axis=1), axis=0)
+    rng.shuffle(tt)
+    v = rng.random(len(tt))
+    say("construction random", N, n, len(tt))
+    t = sp.SpSymtensor(n, N, tt, v)
+    check_tree(t, N, n, tt, v)
+
+
+def test_s3ttmc_contact5_golden(contact5):
+    o, d, inds, vals = contact5
+    t = sp.SpSymtensor(d, o, inds, vals)
+    say("s3ttmc contact-5 R=4")
+    Y = t.s3ttmc(oracle.uniform_factor(1, d, 4))
+    G = np.load(os.path.join(GOLDEN, "contact5_y_r4.npy"))
+    assert werr(Y, G, 4, o) <= 1e-12
+    say("s3ttmc contact-5 R=8")
+    U8 = oracle.uniform_factor(1, d, 8)
+    Y8 = t.s3ttmc(U8)
+    Yo = oracle.Tree(o, d, inds, vals).s3ttmc(U8, 8)
+    assert werr(Y8, Yo, 8, o) <= 1e-12
+
+
+def test_s3ttmc_small_golden():
+    z = np.load(os.path.join(GOLDEN, "small_trees.npz"))
+    for ci in range(5):
+        tt, v = z["c%d_inds" % ci], z["c%d_vals" % ci]
+        N, n = tt.shape[1], int(tt.max())
+        say("s3ttmc golden case", ci)
+        t = sp.SpSymtensor(n, N, tt, v)
+        Y = t.s3ttmc(z["c%d_U" % ci])
+        assert werr(Y, z["c%d_Y" % ci], 3, N) <= 1e-12
+
+
+@pytest.mark.parametrize("N,n,cnt,R,seed", [
+    (3, 40, 500, 1, 1), (3, 40, 500, 7, 2), (4, 25, 800, 2, 3), (4, 60, 3000, 10, 4),
+    (5, 20, 900, 3, 5), (5, 30, 2000, 8, 6), (6, 12, 600, 4, 7), (6, 15, 1500, 8, 8),
+    (7, 9, 400, 3, 9), (7, 10, 800, 6, 10), (8, 6, 200, 3, 11), (4, 7, 300, 16, 12),
+])
+def test_s3ttmc_random_vs_oracle(N, n, cnt, R, seed):
+    rng = np.random.default_rng(seed)
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(cnt, N)), axis=1), axis=0)
+    v = rng.standard_normal(len(tt))
+    U = rng.standard_normal((n, R))
+    say("s3ttmc random", N, n, len(tt), R)
+    t = sp.SpSymtensor(n, N, tt, v)
+    Y = t.s3ttmc(U)
+    Yo = oracle.Tree(N, n, tt, v).s3ttmc(U, R, nthreads=4)
+    assert Y.shape == Yo.shape
+    assert werr(Y, Yo, R, N) <= 1e-12
+    if n <= 12 and N <= 6:
+        Yd = oracle.s3ttmc_dense(N, n, tt, v, U, R)
+        assert werr(Y, Yd, R, N) <= 1e-11
+
+
+def test_s3ttmc_hub_rows_split():
+    # Zipf tensor: hub rows carry far more owners than one work chunk
+    inds, vals = sp.generate(sp.GEN_ZIPF, 4, 2000, 60000, 1.1, 1, seed=3)
+    R = 10
+    U = oracle.random_factor(0, 2000, R)
+    say("s3ttmc zipf hubs", len(vals))
+    t = sp.SpSymtensor(2000, 4, inds, vals)
+    Y = t.s3ttmc(U)
+    Yo = oracle.Tree(4, 2000, inds, vals).s3ttmc(U, R, nthreads=8)
+    assert werr(Y, Yo, R, 4) <= 1e-12
+    # bit-identical repeat (no atomics)
+    t.s3ttmc_dev()
+    assert np.array_equal(t.y(), Y)
+
+
+def test_s3ttmc_fp32(contact5):
+    o, d, inds, vals = contact5
+    t = sp.SpSymtensor(d, o, inds, vals)
+    U = oracle.random_factor(0, d, 8)
+    say("s3ttmc fp32 contact-5")
+    Y32 = t.s3ttmc(U, precision=sp.PREC_FP32)
+    Yo = oracle.Tree(o, d, inds, vals).s3ttmc(U, 8)
+    assert werr(Y32, Yo, 8, o) <= 2e-5
+
+
+def test_dense_ops_vs_numpy():
+    rng = np.random.default_rng(21)
+    N, n, R = 4, 50, 5
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(2500, N)), axis=1), axis=0)
+    v = rng.random(len(tt))
+    t = sp.SpSymtensor(n, N, tt, v)
+    U = np.linalg.qr(rng.standard_normal((n, R)))[0]
+    say("dense ops")
+    Y = t.s3ttmc(U)
+    W = oracle.symtens_coeff(R, N - 1)
+    core = t.gemmtn()
+    assert np.allclose(core, U.T @ Y, rtol=1e-12, atol=1e-12 * np.abs(core).max())
+    cn = t.core_norm()
+    assert np.isclose(cn, np.sqrt((core ** 2 * W).sum()), rtol=1e-13)
+    G = t.gram()
+    sw = np.sqrt(W)
+    Gref = (Y * sw).T @ (Y * sw)
+    assert np.allclose(G, Gref, rtol=1e-11, atol=1e-12 * np.abs(Gref).max())
+    t.symgemm()
+    F = t.factor()
+    assert np.allclose(F, (Y * W) @ core.T, rtol=1e-11, atol=1e-12 * np.abs(F).max())
+    t.qr()
+    Q = t.factor()
+    assert np.allclose(Q.T @ Q, np.eye(R), atol=1e-12)
+    Qref = np.linalg.qr((Y * W) @ core.T)[0]
+    assert np.allclose(np.abs(Q.T @ Qref), np.eye(R), atol=1e-8)
+
+
+def test_leftsvd_subspace():
+    rng = np.random.default_rng(22)
+    N, n, R = 4, 80, 4
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(4000, N)), axis=1), axis=0)
+    v = rng.random(len(tt))
+    t = sp.SpSymtensor(n, N, tt, v)
+    U = np.linalg.qr(rng.standard_normal((n, R)))[0]
+    say("leftsvd")
+    Y = t.s3ttmc(U)
+    sig = t.leftsvd_y()
+    Un = t.factor()
+    Yu = np.stack([oracle.unpack(Y[i], R, N - 1) for i in range(n)])
+    Us, s, _ = np.linalg.svd(Yu, full_matrices=False)
+    assert np.allclose(sig, s[:R], rtol=1e-10)
+    assert np.allclose(Un.T @ Un, np.eye(R), atol=1e-10)
+    # same subspace: singular values of Un^T Us all 1
+    c = np.linalg.svd(Un.T @ Us[:, :R], compute_uv=False)
+    assert np.allclose(c, 1.0, atol=1e-9)
+
+
+def test_regularize_matches_reference_math():
+    rng = np.random.default_rng(23)
+    n, R, delta = 60, 4, 0.1
+    tt = np.array([[1, 2, 3], [2, 3, 4]], np.uint32)
+    t = sp.SpSymtensor(n, 3, tt, np.ones(2))
+    U = rng.standard_normal((n, R))
+    t.set_factor(U)
+    say("regularize")
+    t.regularize(delta)
+    Un = t.factor()
+    # spsymtensor.cpp:168-199: clip rows to norm delta, then left singular vectors
+    nr = np.linalg.norm(U, axis=1, keepdims=True)
+    Uc = np.where(nr > delta, U * (delta / nr), U)
+    Us = np.linalg.svd(Uc, full_matrices=False)[0]
+    assert np.allclose(Un.T @ Un, np.eye(R), atol=1e-12)
+    c = np.linalg.svd(Un.T @ Us, compute_uv=False)
+    assert np.allclose(c, 1.0, atol=1e-10)
+    # columns come out as the singular vectors themselves (up to sign)
+    assert np.allclose(np.abs(np.sum(Un * Us, axis=0)), 1.0, atol=1e-8)

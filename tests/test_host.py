@@ -1,0 +1,118 @@
+"""CPU-side checks of the product library: exports, loader, generators.
+
+No compute call here touches a GPU.
+"""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import symprop_amd as sp
+
+from conftest import GOLDEN, ROOT
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(sp_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(sp.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 25
+    for n in sorted(names):
+        assert hasattr(L, n), n
+    # the Python binding covers the same set
+    assert names <= set(sp.SIGNATURES)
+
+
+def test_version_and_error_string():
+    assert b"gfx950" in sp.lib().sp_version()
+    assert sp.lib().sp_last_error() is not None
+
+
+def test_loader_matches_oracle_loader(tmp_path):
+    path = os.path.join(GOLDEN, "contact-5.tns")
+    a = sp.load_tns(path)
+    b = oracle.load_tns(path)
+    assert a[0] == b[0] and a[1] == b[1]
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    p = tmp_path / "odd.tns"
+    p.write_text("4 10\n4 3 2 1 0.25\n10 1 1 5 -2\n7 7 7\n")
+    a = sp.load_tns(str(p))
+    b = oracle.load_tns(str(p))
+    assert a[2].tolist() == b[2].tolist() == [[1, 2, 3, 4], [1, 1, 5, 10]]
+    assert a[3].tolist() == b[3].tolist() == [0.25, -2.0]
+
+
+def test_loader_missing_file():
+    with pytest.raises(sp.SpError) as e:
+        sp.load_tns("/nonexistent/x.tns")
+    assert e.value.code == sp.SP_ERR_IO
+
+
+@pytest.mark.parametrize("kind,order,dim,nnz,param,k", [
+    (sp.GEN_UNIFORM, 5, 327, 8000, 0.0, 1),
+    (sp.GEN_ZIPF, 4, 1000, 20000, 1.1, 1),
+    (sp.GEN_PLANTED, 6, 4000, 20000, 0.8, 8),
+])
+def test_generators(kind, order, dim, nnz, param, k):
+    inds, vals = sp.generate(kind, order, dim, nnz, param, k, seed=0)
+    inds2, vals2 = sp.generate(kind, order, dim, nnz, param, k, seed=0)
+    assert np.array_equal(inds, inds2) and np.array_equal(vals, vals2)
+    i3, _ = sp.generate(kind, order, dim, nnz, param, k, seed=1)
+    assert not np.array_equal(inds[: len(i3)], i3[: len(inds)])
+    assert inds.min() >= 1 and inds.max() <= dim
+    assert np.all(np.diff(inds.astype(np.int64), axis=1) >= 0)
+    # unique and lexicographically sorted
+    assert len(np.unique(inds, axis=0)) == len(inds)
+    assert np.all(np.lexsort(inds.T[::-1]) == np.arange(len(inds)))
+    if kind == sp.GEN_UNIFORM:
+        assert 0.98 * nnz <= len(inds) <= nnz
+        assert np.all(vals == 1.0)
+    else:
+        assert len(inds) == nnz
+    if kind == sp.GEN_ZIPF:
+        assert np.all((vals >= 0) & (vals < 1))
+        counts = np.bincount(inds.ravel(), minlength=dim + 1)
+        assert counts[1] > counts[10] > counts[500]
+    if kind == sp.GEN_PLANTED:
+        block = dim // k
+        comm = (inds - 1) // block
+        mode = np.array([np.bincount(r, minlength=k).max() for r in comm])
+        assert mode.mean() > 0.7 * order
+
+
+def test_generator_bad_args():
+    with pytest.raises(sp.SpError):
+        sp.generate(7, 3, 10, 10)
+
+
+def test_ctx_create_rejects_bad_input_before_touching_a_gpu():
+    import ctypes as C
+    L = sp.lib()
+    h = C.c_void_p()
+    t = np.array([[2, 1, 3]], np.uint32)
+    v = np.ones(1)
+    rc = L.sp_ctx_create(3, 5, 1, t.ctypes.data_as(sp.u32p), v.ctypes.data_as(sp.f64p),
+                         C.byref(h))
+    assert rc == sp.SP_ERR_ARG and b"sorted" in L.sp_last_error()
+    t = np.array([[1, 2, 9]], np.uint32)
+    rc = L.sp_ctx_create(3, 5, 1, t.ctypes.data_as(sp.u32p), v.ctypes.data_as(sp.f64p),
+                         C.byref(h))
+    assert rc == sp.SP_ERR_ARG
+    t = np.array([[1, 2]], np.uint32)
+    rc = L.sp_ctx_create(2, 5, 1, t.ctypes.data_as(sp.u32p), v.ctypes.data_as(sp.f64p),
+                         C.byref(h))
+    assert rc == sp.SP_ERR_UNSUPPORTED
+    rc = L.sp_ctx_create(3, 5, 1, None, None, C.byref(h))
+    assert rc == sp.SP_ERR_ARG
