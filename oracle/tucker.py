@@ -1,0 +1,103 @@
+"""Oracle Tucker loops (numpy LAPACK) -- TEST INFRASTRUCTURE ONLY.
+
+Restates TuckerHOOI / TuckerHOQRI / TuckerAltHOOI of
+src/sparse/spsymtensor.cpp:201-337 with regularize (168-199), the error
+formula of src/tucker.cpp:40-45 and symtens_norm / symtens_rel_res of
+src/dense/symtensor.cpp:78-106.  LAPACK calls map as in
+src/dense/matrixops.cpp: leftsvd -> numpy.linalg.svd (left vectors, 'O'),
+qr -> numpy.linalg.qr (thin Q), syevd -> numpy.linalg.eigh (ascending,
+last R columns).  S3TTMc comes from the C oracle (liboracle.so).
+Sizes: n x R^(N-1) dense unpacked Y for HOOI, so small tensors only.
+"""
+import numpy as np
+
+import oracle
+
+
+def regularize(U, delta):
+    U = U.copy()
+    nr = np.sqrt((U * U).sum(axis=1))
+    big = delta < nr
+    U[big] *= (delta / nr[big])[:, None]
+    return np.linalg.svd(U, full_matrices=False)[0]
+
+
+def unpack_rows(Y, R, k):
+    return np.stack([oracle.unpack(Y[i], R, k) for i in range(Y.shape[0])])
+
+
+def rel_residual(x, y):
+    return ((x - y) ** 2).sum() / (x ** 2).sum()
+
+
+def hooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
+    N, n = T.order, T.dim
+    norm = oracle.norm(N, T.inds, T.vals)
+    U = oracle.random_factor(seed, n, R) if U0 is None else np.array(U0, float)
+    core_prev = np.zeros((R, R ** (N - 1)))
+    errors = []
+    prev = 0.0
+    for _ in range(max_iter):
+        U = regularize(U, delta)
+        Y = unpack_rows(T.s3ttmc(U, R, nthreads=4), R, N - 1)
+        U = np.linalg.svd(Y, full_matrices=False)[0][:, :R]
+        core = U.T @ Y
+        err = np.sqrt(norm ** 2 - (core ** 2).sum()) / norm
+        errors.append(err)
+        if abs(prev - err) < rel_tol:
+            break
+        prev = err
+        core_prev = core
+    return U, np.array(errors), core
+
+
+def hoqri(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
+    N, n = T.order, T.dim
+    norm = oracle.norm(N, T.inds, T.vals)
+    W = oracle.symtens_coeff(R, N - 1)
+    U = oracle.random_factor(seed, n, R) if U0 is None else np.array(U0, float)
+    errors = []
+    prev = 0.0
+    core = None
+    for _ in range(max_iter):
+        U = regularize(U, delta)
+        Y = T.s3ttmc(U, R, nthreads=4)
+        core = U.T @ Y
+        U = np.linalg.qr((Y * W) @ core.T)[0]
+        cn = np.sqrt((core ** 2 * W).sum())
+        err = np.sqrt(norm ** 2 - cn ** 2) / norm
+        errors.append(err)
+        if abs(prev - err) < rel_tol:
+            break
+        prev = err
+    return U, np.array(errors), core
+
+
+def althooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
+    N, n = T.order, T.dim
+    norm = oracle.norm(N, T.inds, T.vals)
+    W = oracle.symtens_coeff(R, N - 1)
+    U = oracle.random_factor(seed, n, R) if U0 is None else np.array(U0, float)
+    errors = []
+    prev = 0.0
+    core = None
+    for _ in range(max_iter):
+        U = regularize(U, delta)
+        Y = T.s3ttmc(U, R, nthreads=4)
+        A = (Y * W) @ Y.T
+        U = np.linalg.eigh(A)[1][:, n - R:]
+        core = U.T @ Y
+        cn = np.sqrt((core ** 2 * W).sum())
+        err = np.sqrt(norm ** 2 - cn ** 2) / norm
+        errors.append(err)
+        if abs(prev - err) < rel_tol:
+            break
+        prev = err
+    return U, np.array(errors), core
+
+
+def subspace_cos(U1, U2):
+    """Smallest cosine of the principal angles between span(U1), span(U2)."""
+    Q1 = np.linalg.qr(U1)[0]
+    Q2 = np.linalg.qr(U2)[0]
+    return float(np.linalg.svd(Q1.T @ Q2, compute_uv=False).min())

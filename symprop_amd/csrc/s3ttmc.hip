@@ -276,8 +276,24 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
   SP_HIP(hipMemcpy(c.d_W.p, c.tabs[m].weight.data(), c.P * sizeof(double),
                    hipMemcpyHostToDevice));
 
-  // work chunks: enough chunks to fill the GPU several times over
-  const u64 target = std::max<u64>(16, std::min<u64>(2048, c.n_owners / 8192 + 1));
+  // specialised MFMA kernel for this (order, rank), with its fold table
+  c.fast_kernel = fast_kernel_for(m, R, &c.fast_H);
+  if (c.fast_kernel) {
+    std::vector<u32> fptr, fterm;
+    make_fold_table(R, m, c.fast_H, fptr, fterm);
+    c.d_fold_ptr.alloc(fptr.size() * sizeof(u32));
+    c.d_fold_term.alloc(std::max<size_t>(1, fterm.size()) * sizeof(u32));
+    SP_HIP(hipMemcpy(c.d_fold_ptr.p, fptr.data(), fptr.size() * sizeof(u32),
+                     hipMemcpyHostToDevice));
+    SP_HIP(hipMemcpy(c.d_fold_term.p, fterm.data(), fterm.size() * sizeof(u32),
+                     hipMemcpyHostToDevice));
+  }
+
+  // work chunks: enough chunks to fill the GPU several times over; the
+  // 256-thread MFMA kernel takes larger chunks than the one-wave kernel
+  const u64 target =
+      c.fast_kernel ? std::max<u64>(256, std::min<u64>(4096, c.n_owners / 2048 + 1))
+                    : std::max<u64>(16, std::min<u64>(2048, c.n_owners / 8192 + 1));
   build_chunks(c, target);
   c.d_chunks.alloc(c.h_chunks.size() * sizeof(Chunk));
   SP_HIP(hipMemcpy(c.d_chunks.p, c.h_chunks.data(), c.h_chunks.size() * sizeof(Chunk),
@@ -325,7 +341,14 @@ void s3ttmc_run(Ctx &c, int precision) {
   a.partial = c.d_partial.as<double>();
   const unsigned nchunks = (unsigned)c.h_chunks.size();
 
-  if (precision == SP_PREC_FP64) {
+  if (precision == SP_PREC_FP64 && c.fast_kernel) {
+    ScopedTimer t(c, "s3ttmc_mfma64");
+    launch_fast(c.fast_kernel, nchunks, c.stream, c.d_chunks.as<Chunk>(),
+                c.w_tuple.as<u32>(), c.w_coef.as<double>(), c.w_segptr.as<u64>(),
+                c.d_U.as<double>(), c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
+                c.d_fact.as<double>(), c.d_Y.as<double>(), c.d_partial.as<double>());
+    SP_CHECK_LAUNCH();
+  } else if (precision == SP_PREC_FP64) {
     const size_t lds = off * sizeof(double);
     if (lds > 65536) throw Error(SP_ERR_UNSUPPORTED, "packed tensors exceed LDS");
     ScopedTimer t(c, "s3ttmc_generic");

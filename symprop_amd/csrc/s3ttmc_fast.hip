@@ -1,0 +1,349 @@
+// Fast S3TTMc kernels: compile-time (leaf size M = N-1, rank R, head degree H).
+//
+// With the polynomial form of the leaf tensor (see s3ttmc.hip), the packed Y
+// row of index r is
+//     Y[r] = fact o fold( sum_{owners o of r} (c_o * head_o) (x) tail_o ),
+// where head_o is the product of the linear forms of the first H leaf rows
+// (P_H packed coefficients), tail_o that of the last T = M-H rows (P_T), fold
+// adds every (h, t) pair into the packed entry of the merged multi-index and
+// fact multiplies by prod(f_J!).  The owner sum is a GEMM with K = owners:
+//     Ypre (P_H x P_T) = A^T B,   A = [c_o head_o],  B = [tail_o],
+// run on FP64 MFMA (v_mfma_f64_16x16x4f64, 16x16 tiles, K = 4 owners).
+//
+// Work split of a 256-thread workgroup over a batch of 64 owners of one row:
+//   producer: lane = owner, wave w computes the w-th quarter of the head and
+//             tail coefficients in registers (fully unrolled, constant
+//             indices) and stores them to LDS;
+//   consumer: wave w owns output tiles w, w+4, ...; each MFMA reads one A
+//             and one B fragment from LDS.
+// At the end of a row the tiles are written to LDS as Ypre, folded (a CSR
+// table of (h, t) pairs per packed output) and written once to Y.
+#include "sp_internal.hpp"
+
+#include <algorithm>
+
+namespace spamd {
+
+namespace {
+
+constexpr int cbinom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  long long r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return (int)r;
+}
+constexpr int psz(int R, int k) { return cbinom(R + k - 1, k); }
+
+constexpr int pidx(const int *t, int k, int R) {
+  int off = 0, start = 0;
+  for (int j = 0; j < k; ++j) {
+    for (int i = start; i < t[j]; ++i) off += psz(R - i, k - 1 - j);
+    start = t[j];
+  }
+  return off;
+}
+
+// Gather table of the product of a degree-K polynomial with a linear form:
+// output J (degree K+1) = sum over its distinct values j of p[J - e_j] u[j].
+template <int R, int K>
+struct ProdTab {
+  static constexpr int PO = psz(R, K + 1);
+  int n[PO];
+  int src[PO][K + 1];
+  int jj[PO][K + 1];
+  constexpr ProdTab() : n{}, src{}, jj{} {
+    int idx[K + 1] = {};
+    for (int p = 0; p < PO; ++p) {
+      int cnt = 0;
+      for (int q = 0; q <= K; ++q) {
+        if (q > 0 && idx[q] == idx[q - 1]) continue;
+        int sub[K + 1] = {};
+        int w = 0;
+        for (int r = 0; r <= K; ++r)
+          if (r != q) sub[w++] = idx[r];
+        src[p][cnt] = pidx(sub, K, R);
+        jj[p][cnt] = idx[q];
+        ++cnt;
+      }
+      n[p] = cnt;
+      int pos = K;
+      while (pos >= 0 && idx[pos] == R - 1) --pos;
+      if (pos < 0) break;
+      idx[pos]++;
+      for (int q = pos + 1; q <= K; ++q) idx[q] = idx[pos];
+    }
+  }
+};
+
+template <int R, int K>
+__device__ __forceinline__ double prod_entry(const double *p, const double *u, int J) {
+  constexpr ProdTab<R, K> T{};
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < K + 1; ++q)
+    if (q < T.n[J]) s = fma(p[T.src[J][q]], u[T.jj[J][q]], s);
+  return s;
+}
+
+// All coefficients of degree K+1 into registers.
+template <int R, int K>
+__device__ __forceinline__ void prod_full(const double *p, const double *u, double *out) {
+#pragma unroll
+  for (int J = 0; J < psz(R, K + 1); ++J) out[J] = prod_entry<R, K>(p, u, J);
+}
+
+// Quarter W of the degree-D polynomial of rows rows[0..D-1] (each R values),
+// scaled, into dst[J] for J in [lo, hi) of the quarter; zero padding up to
+// padded width (pad entries handled by the last quarter).
+template <int R, int D, int W>
+__device__ __forceinline__ void poly_quarter(const double *rows, double scale, double *dst,
+                                             int padded) {
+  constexpr int P = psz(R, D);
+  constexpr int Q = (P + 3) / 4;
+  constexpr int lo = W * Q;
+  constexpr int hi = (W + 1) * Q < P ? (W + 1) * Q : P;
+  if constexpr (D == 1) {
+#pragma unroll
+    for (int J = lo; J < hi; ++J) dst[J] = rows[J] * scale;
+  } else if constexpr (D == 2) {
+#pragma unroll
+    for (int J = lo; J < hi; ++J) dst[J] = prod_entry<R, 1>(rows, rows + R, J) * scale;
+  } else {
+    static_assert(D == 3, "head/tail degree above 3 is not instantiated");
+    double p2[psz(R, 2)];
+    prod_full<R, 1>(rows, rows + R, p2);
+#pragma unroll
+    for (int J = lo; J < hi; ++J) dst[J] = prod_entry<R, 2>(p2, rows + 2 * R, J) * scale;
+  }
+  if constexpr (W == 3) {
+    for (int J = P; J < padded; ++J) dst[J] = 0.0;
+  }
+}
+
+struct FastArgs {
+  const Chunk *chunks;
+  const u32 *tup;
+  const double *coef;
+  const u64 *segptr;
+  const double *U;
+  const u32 *fold_ptr;   // P_M + 1
+  const u32 *fold_term;  // (h << 16) | t
+  const double *fact;    // prod f_J! of the P_M outputs
+  double *Y;
+  double *partial;
+};
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int M, int R, int H>
+__global__ void __launch_bounds__(256)
+s3ttmc_mfma64(FastArgs a) {
+  constexpr int T = M - H;
+  constexpr int PH = psz(R, H), PT = psz(R, T), PM = psz(R, M);
+  constexpr int TH = (PH + 15) / 16, TT = (PT + 15) / 16;
+  constexpr int NTILE = TH * TT;
+  constexpr int TPW = (NTILE + 3) / 4;
+  constexpr int BO = 64;
+  constexpr int LDA = TH * 16 + 1;  // odd stride in doubles: spreads the 4 row groups over banks
+  constexpr int LDB = TT * 16 + 1;
+  constexpr int SA = BO * LDA, SB = BO * LDB;
+  constexpr int SF = PH * PT;
+  constexpr int SMEM = (SA + SB) > SF ? (SA + SB) : SF;
+  __shared__ double sm[SMEM];
+  double *sA = sm;
+  double *sB = sm + SA;
+  double *sF = sm;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const Chunk ch = a.chunks[blockIdx.x];
+
+  for (u32 r = ch.r_begin; r < ch.r_end; ++r) {
+    u64 ob, oe;
+    if (ch.partial >= 0) {
+      ob = ch.o_begin;
+      oe = ch.o_end;
+    } else {
+      ob = a.segptr[r];
+      oe = a.segptr[r + 1];
+    }
+    d4 acc[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (u64 o0 = ob; o0 < oe; o0 += BO) {
+      const int nb = (int)((oe - o0) < (u64)BO ? (oe - o0) : (u64)BO);
+      const int ksteps = (nb + 3) >> 2;
+      // ---- producer: lane = owner, wave = quarter of the coefficients
+      if (lane < ksteps * 4) {
+        double *arow = sA + lane * LDA;
+        double *brow = sB + lane * LDB;
+        if (lane < nb) {
+          const u64 o = o0 + lane;
+          const u32 *tp = a.tup + o * M;
+          const double c = a.coef[o];
+          double rows[(H > T ? H : T) * R];
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            const double *ur = a.U + (u64)tp[k] * R;
+#pragma unroll
+            for (int j = 0; j < R; ++j) rows[k * R + j] = ur[j];
+          }
+          switch (wave) {
+            case 0: poly_quarter<R, H, 0>(rows, c, arow, TH * 16); break;
+            case 1: poly_quarter<R, H, 1>(rows, c, arow, TH * 16); break;
+            case 2: poly_quarter<R, H, 2>(rows, c, arow, TH * 16); break;
+            default: poly_quarter<R, H, 3>(rows, c, arow, TH * 16); break;
+          }
+#pragma unroll
+          for (int k = 0; k < T; ++k) {
+            const double *ur = a.U + (u64)tp[H + k] * R;
+#pragma unroll
+            for (int j = 0; j < R; ++j) rows[k * R + j] = ur[j];
+          }
+          switch (wave) {
+            case 0: poly_quarter<R, T, 0>(rows, 1.0, brow, TT * 16); break;
+            case 1: poly_quarter<R, T, 1>(rows, 1.0, brow, TT * 16); break;
+            case 2: poly_quarter<R, T, 2>(rows, 1.0, brow, TT * 16); break;
+            default: poly_quarter<R, T, 3>(rows, 1.0, brow, TT * 16); break;
+          }
+        } else {
+          // padding owner of the last K-step: zero row
+          for (int J = wave; J < TH * 16; J += 4) arow[J] = 0.0;
+          for (int J = wave; J < TT * 16; J += 4) brow[J] = 0.0;
+        }
+      }
+      __syncthreads();
+      // ---- consumer: MFMA over the batch
+      const int kr = lane >> 4, kc = lane & 15;
+      for (int ks = 0; ks < ksteps; ++ks) {
+        const double *ar = sA + (ks * 4 + kr) * LDA;
+        const double *br = sB + (ks * 4 + kr) * LDB;
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+          const int tile = wave + 4 * i;
+          if (tile < NTILE) {
+            const int th = tile / TT, tt = tile % TT;
+            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[th * 16 + kc], br[tt * 16 + kc],
+                                                          acc[i], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // ---- fold Ypre into the packed row
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int tile = wave + 4 * i;
+      if (tile < NTILE) {
+        const int th = tile / TT, tt = tile % TT;
+        const int t = tt * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int h = th * 16 + 4 * q + (lane >> 4);
+          if (h < PH && t < PT) sF[h * PT + t] = acc[i][q];
+        }
+      }
+    }
+    __syncthreads();
+    double *out = (ch.partial >= 0) ? a.partial + (u64)ch.partial * PM : a.Y + (u64)r * PM;
+    for (int J = threadIdx.x; J < PM; J += 256) {
+      double s = 0.0;
+      for (u32 q = a.fold_ptr[J]; q < a.fold_ptr[J + 1]; ++q) {
+        const u32 e = a.fold_term[q];
+        s += sF[(e >> 16) * PT + (e & 0xFFFFu)];
+      }
+      out[J] = s * a.fact[J];
+    }
+    __syncthreads();
+  }
+}
+
+using FastKernel = void (*)(FastArgs);
+
+struct FastEntry {
+  int M, R, H;
+  FastKernel k;
+};
+
+#define SP_FAST(M, R, H) FastEntry{M, R, H, s3ttmc_mfma64<M, R, H>}
+
+const FastEntry kFast[] = {
+    // order 3
+    SP_FAST(2, 2, 1), SP_FAST(2, 4, 1), SP_FAST(2, 8, 1), SP_FAST(2, 10, 1),
+    // order 4
+    SP_FAST(3, 2, 2), SP_FAST(3, 3, 2), SP_FAST(3, 4, 2), SP_FAST(3, 8, 2),
+    SP_FAST(3, 10, 2),
+    // order 5
+    SP_FAST(4, 2, 2), SP_FAST(4, 3, 2), SP_FAST(4, 4, 2), SP_FAST(4, 8, 2),
+    // order 6
+    SP_FAST(5, 3, 3), SP_FAST(5, 4, 3), SP_FAST(5, 8, 3),
+    // order 7
+    SP_FAST(6, 3, 3), SP_FAST(6, 6, 3),
+};
+
+}  // namespace
+
+// Fold table of (h, t) pairs per packed output of degree M for head degree H.
+void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term) {
+  const int T = M - H;
+  const PackedTables th = make_tables(R, H), tt = make_tables(R, T), tm = make_tables(R, M);
+  auto enumerate = [&](int k) {
+    std::vector<std::vector<u32>> out;
+    std::vector<u32> idx(k, 0);
+    while (true) {
+      out.push_back(idx);
+      int p = k - 1;
+      while (p >= 0 && idx[p] == R - 1) --p;
+      if (p < 0) break;
+      idx[p]++;
+      for (int q = p + 1; q < k; ++q) idx[q] = idx[p];
+    }
+    return out;
+  };
+  const auto eh = enumerate(H), et = enumerate(T);
+  auto index_of = [&](const std::vector<u32> &t) {
+    u64 off = 0;
+    u32 start = 0;
+    const int k = (int)t.size();
+    for (int j = 0; j < k; ++j) {
+      for (u32 i = start; i < t[j]; ++i) off += symsize(R - i, k - 1 - j);
+      start = t[j];
+    }
+    return off;
+  };
+  std::vector<std::vector<u32>> lists(tm.P);
+  for (u64 h = 0; h < th.P; ++h)
+    for (u64 t = 0; t < tt.P; ++t) {
+      std::vector<u32> J(eh[h]);
+      J.insert(J.end(), et[t].begin(), et[t].end());
+      std::sort(J.begin(), J.end());
+      lists[index_of(J)].push_back((u32)(h << 16) | (u32)t);
+    }
+  ptr.assign(1, 0);
+  term.clear();
+  for (auto &l : lists) {
+    term.insert(term.end(), l.begin(), l.end());
+    ptr.push_back((u32)term.size());
+  }
+}
+
+// Returns the instantiated fast kernel for (M, R) or null; sets H.
+void *fast_kernel_for(int M, u32 R, int *H) {
+  for (const auto &e : kFast)
+    if (e.M == M && (u32)e.R == R) {
+      *H = e.H;
+      return (void *)e.k;
+    }
+  return nullptr;
+}
+
+void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
+                 const u32 *tup, const double *coef, const u64 *segptr, const double *U,
+                 const u32 *fold_ptr, const u32 *fold_term, const double *fact, double *Y,
+                 double *partial) {
+  FastArgs a{chunks, tup, coef, segptr, U, fold_ptr, fold_term, fact, Y, partial};
+  ((FastKernel)kernel)<<<grid, 256, 0, s>>>(a);
+}
+
+}  // namespace spamd

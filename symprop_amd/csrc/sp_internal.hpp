@@ -157,6 +157,11 @@ struct Ctx {
   DevBuf d_tab_ptr, d_tab_term, d_fact;
   std::vector<u64> tab_ptr_off, tab_term_off;
 
+  // compile-time specialised MFMA kernel for (order-1, R), if instantiated
+  void *fast_kernel = nullptr;
+  int fast_H = 0;
+  DevBuf d_fold_ptr, d_fold_term;
+
   // profiling
   bool profile = false;
   std::vector<Timer> timers;
@@ -176,9 +181,15 @@ template <class K>
 void radix_sort(K *keys, K *keys_tmp, u32 *vals, u32 *vals_tmp, u64 n,
                 int bits, hipStream_t s, K **keys_out, u32 **vals_out);
 
-// S3TTMc (s3ttmc.hip)
+// S3TTMc (s3ttmc.hip, s3ttmc_fast.hip)
 void s3ttmc_prepare(Ctx &c, u32 R);
 void s3ttmc_run(Ctx &c, int precision);
+void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
+void *fast_kernel_for(int M, u32 R, int *H);
+void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
+                 const u32 *tup, const double *coef, const u64 *segptr, const double *U,
+                 const u32 *fold_ptr, const u32 *fold_term, const double *fact, double *Y,
+                 double *partial);
 
 // dense Y-side ops (dense.hip)
 void dense_gemmtn(Ctx &c);

@@ -1,0 +1,99 @@
+"""GPU Tucker loops (HOOI / HOQRI / AltHOOI) against the oracle's loops.
+
+Both start from the same libstdc++ mt19937 + normal_distribution factor, so
+the reported error trajectories must agree (they depend on subspaces only,
+not on the sign/rotation choices of the eigen/QR/SVD routines).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import tucker
+import symprop_amd as sp
+
+from conftest import GOLDEN
+from test_oracle import exact_rank_tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+ALGS = [(0, tucker.hooi), (1, tucker.hoqri), (2, tucker.althooi)]
+
+
+@pytest.mark.parametrize("alg,ofn", ALGS)
+def test_trajectory_matches_oracle(alg, ofn):
+    rng = np.random.default_rng(7)
+    N, n, R = 5, 40, 3
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(700, N)), axis=1), axis=0)
+    v = rng.random(len(tt))
+    say("tucker alg", alg, N, n, len(tt), R)
+    t = sp.SpSymtensor(n, N, tt, v)
+    r = t.tucker(alg=alg, rank=R, max_iter=8, rel_tol=1e-14, delta=0.5, seed=0)
+    Uo, eo, _ = ofn(oracle.Tree(N, n, tt, v), R, 8, rel_tol=1e-14, delta=0.5, seed=0)
+    assert r["iters"] == len(eo)
+    assert np.allclose(r["errors"], eo, rtol=0, atol=1e-10)
+    assert tucker.subspace_cos(r["U"], Uo) > 1 - 1e-9
+    assert np.allclose(r["U"].T @ r["U"], np.eye(R), atol=1e-10)
+
+
+def test_exact_rank_recovered():
+    tt, v, X, U0 = exact_rank_tensor(n=9, R=2, N=4, seed=1)
+    say("tucker exact rank")
+    t = sp.SpSymtensor(9, 4, tt, v)
+    r = t.tucker(alg=0, rank=2, max_iter=10, rel_tol=1e-12, delta=100.0, seed=0)
+    assert r["errors"][-1] < 1e-6
+    assert tucker.subspace_cos(r["U"], U0) > 1 - 1e-8
+
+
+def test_contact5_readme_run(contact5):
+    # README.md:100: symtucker -f contact-5.tns -r 8 -i 20 -a 0 -d 0.1
+    o, d, inds, vals = contact5
+    say("tucker contact-5 HOOI r8 i20 d0.1")
+    t = sp.SpSymtensor(d, o, inds, vals)
+    r = t.tucker(alg=0, rank=8, max_iter=20, rel_tol=1e-6, delta=0.1, seed=0)
+    Uo, eo, _ = tucker.hooi(oracle.Tree(o, d, inds, vals), 8, 20, rel_tol=1e-6,
+                            delta=0.1, seed=0)
+    assert r["iters"] == len(eo)
+    assert np.allclose(r["errors"], eo, rtol=0, atol=1e-9)
+    assert tucker.subspace_cos(r["U"], Uo) > 1 - 1e-7
+
+
+@pytest.mark.parametrize("prec", [sp.PREC_FP32, sp.PREC_BF16])
+def test_mixed_precision_reaches_fp64_fit(prec, contact5):
+    o, d, inds, vals = contact5
+    t = sp.SpSymtensor(d, o, inds, vals)
+    say("tucker mixed", prec)
+    r64 = t.tucker(alg=0, rank=8, max_iter=40, rel_tol=1e-8, delta=0.1, seed=0)
+    rmx = t.tucker(alg=0, rank=8, max_iter=40, rel_tol=1e-8, delta=0.1, seed=0,
+                   precision=prec)
+    assert rmx["fp64_sweeps"] >= 1
+    e64, emx = r64["errors"][-1], rmx["errors"][-1]
+    assert abs(emx - e64) / e64 <= 1e-6
+
+
+def test_cli_runs_and_writes_reference_format(tmp_path):
+    import subprocess
+    out = tmp_path / "out.txt"
+    say("cli")
+    p = subprocess.run([sp.CLI_PATH, "-f", os.path.join(GOLDEN, "contact-5.tns"), "-r", "4",
+                        "-i", "3", "-a", "0", "-d", "0.1", "-o", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "Finished creating compressed dtree" in p.stdout
+    assert "Insertion time:" in p.stdout and "Running HOOI algorithm" in p.stdout
+    assert "Iteration: 0 relative residual:" in p.stdout
+    assert "Accumulated function times:" in p.stdout
+    lines = out.read_text().splitlines()
+    assert lines[0] == "Mode-1 matricized C:"
+    assert lines[5] == "Factor U:"
+    assert len(lines) == 1 + 4 + 1 + 245
+    assert all(l.endswith(",") for l in lines[1:5] + lines[6:])
+    assert len(lines[1].split(",")) == 4 ** 4 + 1

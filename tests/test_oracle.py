@@ -314,3 +314,63 @@ def test_random_factor_is_libstdcxx_normal():
     assert not np.array_equal(a, oracle.random_factor(1, 3, 2))
     big = oracle.random_factor(0, 20000, 5)
     assert abs(big.mean()) < 0.02 and abs(big.std() - 1.0) < 0.02
+
+
+# ---- Tucker loops ----------------------------------------------------------
+
+def exact_rank_tensor(n=7, R=2, N=3, seed=0):
+    """Dense symmetric tensor with multilinear rank R, as unique sorted tuples."""
+    rng = np.random.default_rng(seed)
+    U0 = np.linalg.qr(rng.standard_normal((n, R)))[0]
+    C = rng.standard_normal((R,) * N)
+    C = sum(np.transpose(C, p) for p in itertools.permutations(range(N)))
+    X = C
+    for _ in range(N):
+        X = np.tensordot(X, U0, axes=([0], [1]))
+    tuples = np.array(list(itertools.combinations_with_replacement(range(1, n + 1), N)),
+                      np.uint32)
+    vals = np.array([X[tuple(t - 1)] for t in tuples])
+    return tuples, vals, X, U0
+
+
+def dense_fit_error(X, U):
+    Xc = X
+    for _ in range(X.ndim):
+        Xc = np.tensordot(Xc, U, axes=([0], [0]))
+    return np.sqrt(max((X ** 2).sum() - (Xc ** 2).sum(), 0.0)) / np.sqrt((X ** 2).sum())
+
+
+def test_tucker_oracle_recovers_exact_rank():
+    from oracle import tucker
+    t, v, X, U0 = exact_rank_tensor()
+    T = oracle.Tree(3, 7, t, v)
+    assert np.isclose(oracle.norm(3, t, v), np.sqrt((X ** 2).sum()), rtol=1e-13)
+    for alg, iters, tol in [(tucker.hooi, 10, 1e-6), (tucker.althooi, 10, 1e-6),
+                            (tucker.hoqri, 200, 1e-4)]:
+        U, errs, _ = alg(T, 2, iters, rel_tol=1e-12, delta=100.0)
+        assert errs[-1] < tol, (alg.__name__, errs[-1])
+        assert tucker.subspace_cos(U, U0) > 1 - 1e-6
+        assert np.isclose(dense_fit_error(X, U), errs[-1], atol=1e-6)
+
+
+def test_tucker_oracle_error_formula_matches_dense():
+    from oracle import tucker
+    rng = np.random.default_rng(4)
+    n, N, R = 6, 4, 2
+    t = np.unique(np.sort(rng.integers(1, n + 1, size=(60, N)), axis=1), axis=0)
+    v = rng.random(len(t))
+    T = oracle.Tree(N, n, t, v)
+    X = np.zeros((n,) * N)
+    for tt, vv in zip(t, v):
+        for p in set(itertools.permutations(tt - 1)):
+            X[p] = vv
+    # one iteration from a given factor: the reported error contracts mode 1
+    # with the new factor and modes 2..N with the regularised input factor
+    U_in = rng.standard_normal((n, R))
+    U, errs, _ = tucker.hooi(T, R, 1, rel_tol=0, delta=0.5, U0=U_in)
+    U_reg = tucker.regularize(U_in, 0.5)
+    Xc = np.tensordot(X, U, axes=([0], [0]))
+    for _ in range(N - 1):
+        Xc = np.tensordot(Xc, U_reg, axes=([0], [0]))
+    expect = np.sqrt((X ** 2).sum() - (Xc ** 2).sum()) / np.sqrt((X ** 2).sum())
+    assert np.isclose(expect, errs[-1], rtol=1e-10)
