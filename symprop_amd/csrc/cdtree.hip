@@ -110,16 +110,15 @@ __device__ inline u64 lower_bound_key(const K *__restrict__ a, u64 n, K x) {
   return lo;
 }
 
-// children counts per parent: parent key = child key >> b.
+// CSR from parents to children: the children of parent key k are the child
+// keys in [k << b, (k+1) << b), so ptr[i] = lower_bound(children, key_i << b)
+// (children with smaller prefixes sort first).  ptr[np] = nc.
 template <class KC, class KP>
-__global__ void count_children(const KC *__restrict__ child, u64 nc, int b,
-                               const KP *__restrict__ parent, u64 np,
-                               u64 *__restrict__ counts) {
+__global__ void child_ptr(const KC *__restrict__ child, u64 nc, int b,
+                          const KP *__restrict__ parent, u64 np, u64 *__restrict__ ptr) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
-  const KP pk = (KP)(child[i] >> b);
-  const u64 p = lower_bound_key<KP>(parent, np, pk);
-  atomicAdd((unsigned long long *)&counts[p], 1ull);
+  if (i > np) return;
+  ptr[i] = (i == np) ? nc : lower_bound_key<KC>(child, nc, (KC)parent[i] << b);
 }
 
 // dep_list: node j (s fields) minus position k, searched in the level below.
@@ -214,11 +213,13 @@ __global__ void rem_keys(const u32 *__restrict__ rem, u64 no, u64 *__restrict__ 
   id[i] = (u32)i;
 }
 
-__global__ void seg_counts(const u64 *__restrict__ sorted_rem, u64 no,
-                           u64 *__restrict__ counts) {
-  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= no) return;
-  atomicAdd((unsigned long long *)&counts[sorted_rem[i] - 1], 1ull);
+// Row segments of the rem-sorted owners: segptr[r] = first owner with
+// rem >= r+1 (rem is 1-based), segptr[dim] = owners.
+__global__ void seg_bounds(const u64 *__restrict__ sorted_rem, u64 no, u32 dim,
+                           u64 *__restrict__ segptr) {
+  const u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > dim) return;
+  segptr[r] = (r == dim) ? no : lower_bound_key<u64>(sorted_rem, no, r + 1);
 }
 
 std::vector<int> combinations(int N, int s) {
@@ -342,13 +343,9 @@ void link_levels(Ctx &c, int l) {
   DevLevel &P = c.levels[l];
   DevLevel &C = c.levels[l + 1];
   P.level_ptr.alloc((P.size + 1) * sizeof(u64));
-  SP_HIP(hipMemsetAsync(P.level_ptr.p, 0, (P.size + 1) * sizeof(u64), c.stream));
-  count_children<KC, KP><<<grid_for(C.size, BLK), BLK, 0, c.stream>>>(
-      C.keys.as<KC>(), C.size, c.bits, P.keys.as<KP>(), P.size,
-      P.level_ptr.as<u64>());
+  child_ptr<KC, KP><<<grid_for(P.size + 1, BLK), BLK, 0, c.stream>>>(
+      C.keys.as<KC>(), C.size, c.bits, P.keys.as<KP>(), P.size, P.level_ptr.as<u64>());
   SP_CHECK_LAUNCH();
-  exclusive_scan_u64(P.level_ptr.as<u64>(), P.level_ptr.as<u64>(), P.size + 1,
-                     c.stream);
   // dep_list of level l+1 against level l
   const int s = l + 2;
   C.dep_list.alloc(C.size * (u64)s * sizeof(u64));
@@ -381,11 +378,9 @@ void build_worklist_t(Ctx &c, DevBuf &own_leaf) {
       c.w_coef.as<double>(), c.w_coef32.as<float>());
   SP_CHECK_LAUNCH();
   c.w_segptr.alloc((c.dim + 1) * sizeof(u64));
-  SP_HIP(hipMemsetAsync(c.w_segptr.p, 0, (c.dim + 1) * sizeof(u64), c.stream));
-  seg_counts<<<grid_for(no, BLK), BLK, 0, c.stream>>>(srem, no, c.w_segptr.as<u64>());
+  seg_bounds<<<grid_for((u64)c.dim + 1, BLK), BLK, 0, c.stream>>>(srem, no, c.dim,
+                                                                  c.w_segptr.as<u64>());
   SP_CHECK_LAUNCH();
-  exclusive_scan_u64(c.w_segptr.as<u64>(), c.w_segptr.as<u64>(), c.dim + 1,
-                     c.stream);
   c.h_segptr.resize(c.dim + 1);
   SP_HIP(hipMemcpyAsync(c.h_segptr.data(), c.w_segptr.p, (c.dim + 1) * sizeof(u64),
                         hipMemcpyDeviceToHost, c.stream));

@@ -188,12 +188,20 @@ s3ttmc_generic(GenArgs a, const T *__restrict__ U, const T *__restrict__ coef) {
 __global__ void reduce_split_rows(const SplitRow *__restrict__ split,
                                   const double *__restrict__ partial, u64 P,
                                   double *__restrict__ Y) {
+  // grid: (split rows, column blocks of 64); fixed summation order
   const SplitRow s = split[blockIdx.x];
-  for (u64 J = threadIdx.x; J < P; J += blockDim.x) {
-    double v = 0.0;
-    for (u32 q = s.slot_begin; q < s.slot_end; ++q) v += partial[(u64)q * P + J];
-    Y[(u64)s.row * P + J] = v;
+  const u64 J = (u64)blockIdx.y * 64 + threadIdx.x;
+  if (J >= P) return;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  u32 q = s.slot_begin;
+  for (; q + 4 <= s.slot_end; q += 4) {
+    v0 += partial[(u64)q * P + J];
+    v1 += partial[(u64)(q + 1) * P + J];
+    v2 += partial[(u64)(q + 2) * P + J];
+    v3 += partial[(u64)(q + 3) * P + J];
   }
+  for (; q < s.slot_end; ++q) v0 += partial[(u64)q * P + J];
+  Y[(u64)s.row * P + J] = (v0 + v1) + (v2 + v3);
 }
 
 template <class T>
@@ -202,9 +210,25 @@ __global__ void convert_factor(const double *__restrict__ U, u64 n, T *__restric
   if (i < n) out[i] = (T)U[i];
 }
 
+// Batches of up to 64 owners of one row for the MFMA kernel.
+void push_batches(std::vector<Batch> &out, u32 row, u64 ob, u64 oe) {
+  auto mk = [&](u64 o, u32 nb, bool last) {
+    out.push_back(Batch{(u32)o, (u32)(o >> 32), row, nb | (last ? 0x80000000u : 0u)});
+  };
+  if (ob == oe) {
+    mk(ob, 0, true);
+    return;
+  }
+  for (u64 o = ob; o < oe; o += 64) {
+    const u64 e = std::min(oe, o + 64);
+    mk(o, (u32)(e - o), e == oe);
+  }
+}
+
 void build_chunks(Ctx &c, u64 target) {
   c.h_chunks.clear();
   c.h_split.clear();
+  c.h_batches.clear();
   const auto &sp = c.h_segptr;
   const u32 n = c.dim;
   u64 slot = 0;
@@ -216,7 +240,9 @@ void build_chunks(Ctx &c, u64 target) {
       SplitRow s{r, (u32)slot, 0};
       for (u64 o = sp[r]; o < sp[r + 1]; o += target) {
         const u64 e = std::min(sp[r + 1], o + target);
-        c.h_chunks.push_back(Chunk{o, e, r, r + 1, (i64)slot++});
+        const u64 b0 = c.h_batches.size();
+        push_batches(c.h_batches, r, o, e);
+        c.h_chunks.push_back(Chunk{o, e, r, r + 1, (i64)slot++, b0, c.h_batches.size()});
       }
       s.slot_end = (u32)slot;
       c.h_split.push_back(s);
@@ -225,13 +251,15 @@ void build_chunks(Ctx &c, u64 target) {
     }
     const u32 start = r;
     u64 acc = 0;
+    const u64 b0 = c.h_batches.size();
     while (r < n && r - start < MAX_ROWS) {
       const u64 k = sp[r + 1] - sp[r];
       if (k > target || (acc + k > target && r > start)) break;
       acc += k;
+      push_batches(c.h_batches, r, sp[r], sp[r + 1]);
       ++r;
     }
-    c.h_chunks.push_back(Chunk{sp[start], sp[r], start, r, -1});
+    c.h_chunks.push_back(Chunk{sp[start], sp[r], start, r, -1, b0, c.h_batches.size()});
   }
   c.n_partial = slot;
 }
@@ -298,6 +326,9 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
   c.d_chunks.alloc(c.h_chunks.size() * sizeof(Chunk));
   SP_HIP(hipMemcpy(c.d_chunks.p, c.h_chunks.data(), c.h_chunks.size() * sizeof(Chunk),
                    hipMemcpyHostToDevice));
+  c.d_batches.alloc(std::max<size_t>(1, c.h_batches.size()) * sizeof(Batch));
+  SP_HIP(hipMemcpy(c.d_batches.p, c.h_batches.data(), c.h_batches.size() * sizeof(Batch),
+                   hipMemcpyHostToDevice));
   c.d_split.alloc(std::max<size_t>(1, c.h_split.size()) * sizeof(SplitRow));
   if (!c.h_split.empty())
     SP_HIP(hipMemcpy(c.d_split.p, c.h_split.data(), c.h_split.size() * sizeof(SplitRow),
@@ -344,7 +375,7 @@ void s3ttmc_run(Ctx &c, int precision) {
   if (precision == SP_PREC_FP64 && c.fast_kernel) {
     ScopedTimer t(c, "s3ttmc_mfma64");
     launch_fast(c.fast_kernel, nchunks, c.stream, c.d_chunks.as<Chunk>(),
-                c.w_tuple.as<u32>(), c.w_coef.as<double>(), c.w_segptr.as<u64>(),
+                c.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(),
                 c.d_U.as<double>(), c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
                 c.d_fact.as<double>(), c.d_Y.as<double>(), c.d_partial.as<double>());
     SP_CHECK_LAUNCH();
@@ -369,7 +400,8 @@ void s3ttmc_run(Ctx &c, int precision) {
   }
   if (!c.h_split.empty()) {
     ScopedTimer t(c, "reduce_split_rows");
-    reduce_split_rows<<<(unsigned)c.h_split.size(), 256, 0, c.stream>>>(
+    reduce_split_rows<<<dim3((unsigned)c.h_split.size(), (unsigned)((c.P + 63) / 64)), 64, 0,
+                        c.stream>>>(
         c.d_split.as<SplitRow>(), c.d_partial.as<double>(), c.P, c.d_Y.as<double>());
     SP_CHECK_LAUNCH();
   }

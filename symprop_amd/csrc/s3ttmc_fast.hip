@@ -122,9 +122,9 @@ __device__ __forceinline__ void poly_quarter(const double *rows, double scale, d
 
 struct FastArgs {
   const Chunk *chunks;
+  const Batch *batches;
   const u32 *tup;
   const double *coef;
-  const u64 *segptr;
   const double *U;
   const u32 *fold_ptr;   // P_M + 1
   const u32 *fold_term;  // (h << 16) | t
@@ -134,6 +134,52 @@ struct FastArgs {
 };
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct BatchV {
+  u64 o0;
+  u32 row;
+  int nb;
+  bool last;
+};
+
+// Uniform batch descriptor.  readfirstlane keeps every field in scalar
+// registers at the point of the load, so the load is not sunk into the
+// divergent owner branch (where it would become a vector load whose wait
+// also drains the row prefetch).
+__device__ __forceinline__ BatchV read_batch(const Batch *__restrict__ bs, u64 b) {
+  const Batch x = bs[b];
+  const u32 lo = __builtin_amdgcn_readfirstlane(x.o0_lo);
+  const u32 hi = __builtin_amdgcn_readfirstlane(x.o0_hi);
+  const u32 row = __builtin_amdgcn_readfirstlane(x.row);
+  const u32 nl = __builtin_amdgcn_readfirstlane(x.nb_last);
+  return BatchV{((u64)hi << 32) | lo, row, (int)(nl & 0xFFFFu), (nl >> 31) != 0};
+}
+
+// Loads the owner of this lane in batch `bt` (tuple and coefficient) without
+// a branch, so the loads stay in flight until first use: lanes past the
+// batch read a valid owner (the batch's first, or owner 0 for an empty
+// batch) and get coefficient 0.
+template <int M>
+__device__ __forceinline__ void load_owner(const FastArgs &a, const BatchV &bt, int lane,
+                                           u32 (&tp)[M], double &cv, bool &in) {
+  in = lane < bt.nb;
+  const u64 o = in ? bt.o0 + lane : (bt.nb > 0 ? bt.o0 : 0);
+  const u32 *t = a.tup + o * M;
+#pragma unroll
+  for (int k = 0; k < M; ++k) tp[k] = t[k];
+  cv = a.coef[o];
+}
+
+template <int M, int R>
+__device__ __forceinline__ void load_rows(const double *U, const u32 (&tp)[M],
+                                          double (&rows)[M * R]) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const double *ur = U + (u64)tp[k] * R;
+#pragma unroll
+    for (int j = 0; j < R; ++j) rows[k * R + j] = ur[j];
+  }
+}
 
 template <int M, int R, int H>
 __global__ void __launch_bounds__(256)
@@ -155,98 +201,104 @@ s3ttmc_mfma64(FastArgs a) {
   double *sF = sm;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int kr = lane >> 4, kc = lane & 15;
   const Chunk ch = a.chunks[blockIdx.x];
+  if (ch.b_begin >= ch.b_end) return;
 
-  for (u32 r = ch.r_begin; r < ch.r_end; ++r) {
-    u64 ob, oe;
-    if (ch.partial >= 0) {
-      ob = ch.o_begin;
-      oe = ch.o_end;
-    } else {
-      ob = a.segptr[r];
-      oe = a.segptr[r + 1];
-    }
-    d4 acc[TPW];
+  // LDS offsets of this lane's A and B fragments for the wave's tiles
+  int aoff[TPW], boff[TPW];
 #pragma unroll
-    for (int i = 0; i < TPW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < TPW; ++i) {
+    const int tile = wave + 4 * i < NTILE ? wave + 4 * i : NTILE - 1;
+    aoff[i] = kr * LDA + (tile / TT) * 16 + kc;
+    boff[i] = kr * LDB + (tile % TT) * 16 + kc;
+  }
 
-    for (u64 o0 = ob; o0 < oe; o0 += BO) {
-      const int nb = (int)((oe - o0) < (u64)BO ? (oe - o0) : (u64)BO);
-      const int ksteps = (nb + 3) >> 2;
-      // ---- producer: lane = owner, wave = quarter of the coefficients
-      if (lane < ksteps * 4) {
-        double *arow = sA + lane * LDA;
-        double *brow = sB + lane * LDB;
-        if (lane < nb) {
-          const u64 o = o0 + lane;
-          const u32 *tp = a.tup + o * M;
-          const double c = a.coef[o];
-          double rows[(H > T ? H : T) * R];
+  // Software pipeline (loop-carried registers):
+  //   rows, cur_c      U rows and coefficient of batch b
+  //   tpn, nxt_cv/in   tuple and raw coefficient of batch b+1
+  // The loads for b+1 (rows) and b+2 (tuple) are issued after the first
+  // barrier of iteration b and first used in iteration b+1, so they are in
+  // flight during the MFMA phase.  Batch indices past the chunk are clamped
+  // to its last batch (valid memory, results unused).
+  const u64 blast = ch.b_end - 1;
+  u32 tpn[M];
+  double cur_c, nxt_cv;
+  bool nxt_in;
+  double rows[M * R];
+  {
+    u32 tp0[M];
+    double cv0;
+    bool in0;
+    load_owner<M>(a, read_batch(a.batches, ch.b_begin), lane, tp0, cv0, in0);
+    cur_c = in0 ? cv0 : 0.0;
+    load_rows<M, R>(a.U, tp0, rows);
+    load_owner<M>(a, read_batch(a.batches, min(ch.b_begin + 1, blast)), lane, tpn, nxt_cv,
+                  nxt_in);
+  }
+
+  d4 acc[TPW];
 #pragma unroll
-          for (int k = 0; k < H; ++k) {
-            const double *ur = a.U + (u64)tp[k] * R;
-#pragma unroll
-            for (int j = 0; j < R; ++j) rows[k * R + j] = ur[j];
-          }
-          switch (wave) {
-            case 0: poly_quarter<R, H, 0>(rows, c, arow, TH * 16); break;
-            case 1: poly_quarter<R, H, 1>(rows, c, arow, TH * 16); break;
-            case 2: poly_quarter<R, H, 2>(rows, c, arow, TH * 16); break;
-            default: poly_quarter<R, H, 3>(rows, c, arow, TH * 16); break;
-          }
-#pragma unroll
-          for (int k = 0; k < T; ++k) {
-            const double *ur = a.U + (u64)tp[H + k] * R;
-#pragma unroll
-            for (int j = 0; j < R; ++j) rows[k * R + j] = ur[j];
-          }
-          switch (wave) {
-            case 0: poly_quarter<R, T, 0>(rows, 1.0, brow, TT * 16); break;
-            case 1: poly_quarter<R, T, 1>(rows, 1.0, brow, TT * 16); break;
-            case 2: poly_quarter<R, T, 2>(rows, 1.0, brow, TT * 16); break;
-            default: poly_quarter<R, T, 3>(rows, 1.0, brow, TT * 16); break;
-          }
-        } else {
-          // padding owner of the last K-step: zero row
-          for (int J = wave; J < TH * 16; J += 4) arow[J] = 0.0;
-          for (int J = wave; J < TT * 16; J += 4) brow[J] = 0.0;
-        }
+  for (int i = 0; i < TPW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+
+  for (u64 b = ch.b_begin; b < ch.b_end; ++b) {
+    const BatchV bt = read_batch(a.batches, b);
+    const int nb = bt.nb;
+    const int ksteps = (nb + 3) >> 2;
+    // ---- producer: lane = owner, wave = quarter of the coefficients
+    if (lane < ksteps * 4) {
+      double *arow = sA + lane * LDA;
+      double *brow = sB + lane * LDB;
+      // padding lanes (lane >= nb) write zero rows: both scales are 0
+      const double c = cur_c;
+      const double one = lane < nb ? 1.0 : 0.0;
+      switch (wave) {
+        case 0: poly_quarter<R, H, 0>(rows, c, arow, TH * 16);
+                poly_quarter<R, T, 0>(rows + H * R, one, brow, TT * 16); break;
+        case 1: poly_quarter<R, H, 1>(rows, c, arow, TH * 16);
+                poly_quarter<R, T, 1>(rows + H * R, one, brow, TT * 16); break;
+        case 2: poly_quarter<R, H, 2>(rows, c, arow, TH * 16);
+                poly_quarter<R, T, 2>(rows + H * R, one, brow, TT * 16); break;
+        default: poly_quarter<R, H, 3>(rows, c, arow, TH * 16);
+                 poly_quarter<R, T, 3>(rows + H * R, one, brow, TT * 16); break;
       }
-      __syncthreads();
-      // ---- consumer: MFMA over the batch
-      const int kr = lane >> 4, kc = lane & 15;
-      for (int ks = 0; ks < ksteps; ++ks) {
-        const double *ar = sA + (ks * 4 + kr) * LDA;
-        const double *br = sB + (ks * 4 + kr) * LDB;
-#pragma unroll
-        for (int i = 0; i < TPW; ++i) {
-          const int tile = wave + 4 * i;
-          if (tile < NTILE) {
-            const int th = tile / TT, tt = tile % TT;
-            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[th * 16 + kc], br[tt * 16 + kc],
-                                                          acc[i], 0, 0, 0);
-          }
-        }
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    // ---- prefetch: U rows of batch b+1, tuple of batch b+2
+    cur_c = nxt_in ? nxt_cv : 0.0;
+    load_rows<M, R>(a.U, tpn, rows);
+    load_owner<M>(a, read_batch(a.batches, min(b + 2, blast)), lane, tpn, nxt_cv, nxt_in);
+    // ---- consumer: MFMA over the batch
+    for (int ks = 0; ks < ksteps; ++ks) {
+      const double *ar = sA + ks * 4 * LDA;
+      const double *br = sB + ks * 4 * LDB;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        if (NTILE % 4 == 0 || wave + 4 * i < NTILE)
+          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[aoff[i]], br[boff[i]], acc[i], 0,
+                                                        0, 0);
+      }
+    }
+    __syncthreads();
+    if (!bt.last) continue;
     // ---- fold Ypre into the packed row
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
       const int tile = wave + 4 * i;
       if (tile < NTILE) {
         const int th = tile / TT, tt = tile % TT;
-        const int t = tt * 16 + (lane & 15);
+        const int t = tt * 16 + kc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int h = th * 16 + 4 * q + (lane >> 4);
+          const int h = th * 16 + 4 * q + kr;
           if (h < PH && t < PT) sF[h * PT + t] = acc[i][q];
         }
       }
+      acc[i] = d4{0.0, 0.0, 0.0, 0.0};
     }
     __syncthreads();
-    double *out = (ch.partial >= 0) ? a.partial + (u64)ch.partial * PM : a.Y + (u64)r * PM;
+    double *out = (ch.partial >= 0) ? a.partial + (u64)ch.partial * PM : a.Y + (u64)bt.row * PM;
     for (int J = threadIdx.x; J < PM; J += 256) {
       double s = 0.0;
       for (u32 q = a.fold_ptr[J]; q < a.fold_ptr[J + 1]; ++q) {
@@ -339,10 +391,10 @@ void *fast_kernel_for(int M, u32 R, int *H) {
 }
 
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
-                 const u32 *tup, const double *coef, const u64 *segptr, const double *U,
+                 const Batch *batches, const u32 *tup, const double *coef, const double *U,
                  const u32 *fold_ptr, const u32 *fold_term, const double *fact, double *Y,
                  double *partial) {
-  FastArgs a{chunks, tup, coef, segptr, U, fold_ptr, fold_term, fact, Y, partial};
+  FastArgs a{chunks, batches, tup, coef, U, fold_ptr, fold_term, fact, Y, partial};
   ((FastKernel)kernel)<<<grid, 256, 0, s>>>(a);
 }
 

@@ -13,6 +13,7 @@
 
 namespace spamd {
 
+using u16 = uint16_t;
 using u32 = uint32_t;
 using u64 = uint64_t;
 using i64 = int64_t;
@@ -93,6 +94,16 @@ struct Chunk {
   u64 o_begin, o_end;  // owner range (rem-sorted order)
   u32 r_begin, r_end;  // Y rows written by this chunk
   i64 partial;         // >= 0: piece of one row, result goes to partial slot
+  u64 b_begin, b_end;  // batch range (MFMA kernel)
+};
+
+// Up to 64 consecutive owners of one Y row; `last` marks the batch after
+// which the row is complete (an empty row has one batch with nb = 0).
+// All fields are dwords so that the kernel reads a batch with scalar loads.
+struct Batch {
+  u32 o0_lo, o0_hi;
+  u32 row;
+  u32 nb_last;  // nb in bits 0..15, last in bit 31
 };
 
 struct SplitRow {
@@ -148,7 +159,8 @@ struct Ctx {
   // chunking of the work list for the current R
   std::vector<Chunk> h_chunks;
   std::vector<SplitRow> h_split;
-  DevBuf d_chunks, d_split;
+  std::vector<Batch> h_batches;
+  DevBuf d_chunks, d_split, d_batches;
   u64 n_partial = 0;
   u32 chunk_R = 0;
 
@@ -187,7 +199,7 @@ void s3ttmc_run(Ctx &c, int precision);
 void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
 void *fast_kernel_for(int M, u32 R, int *H);
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
-                 const u32 *tup, const double *coef, const u64 *segptr, const double *U,
+                 const Batch *batches, const u32 *tup, const double *coef, const double *U,
                  const u32 *fold_ptr, const u32 *fold_term, const double *fact, double *Y,
                  double *partial);
 

@@ -154,6 +154,19 @@ def test_s3ttmc_hub_rows_split():
     assert np.array_equal(t.y(), Y)
 
 
+def test_s3ttmc_hub_rows_split_order6():
+    inds, vals = sp.generate(sp.GEN_ZIPF, 6, 600, 20000, 1.2, 1, seed=5)
+    R = 8
+    U = oracle.random_factor(1, 600, R)
+    say("s3ttmc zipf order 6 hubs", len(vals))
+    t = sp.SpSymtensor(600, 6, inds, vals)
+    Y = t.s3ttmc(U)
+    Yo = oracle.Tree(6, 600, inds, vals).s3ttmc(U, R, nthreads=8)
+    assert werr(Y, Yo, R, 6) <= 1e-12
+    t.s3ttmc_dev()
+    assert np.array_equal(t.y(), Y)
+
+
 def test_s3ttmc_fp32(contact5):
     o, d, inds, vals = contact5
     t = sp.SpSymtensor(d, o, inds, vals)
