@@ -69,6 +69,19 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def measured_traffic(kernel, workload_name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this
+    kernel and workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s.md" % (kernel, workload_name))
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def cpu_baseline(inds, vals, order, dim, R, budget_s=25.0):
     """Reference construction + opt5 on the host cores (oracle/_ref)."""
     try:
@@ -190,6 +203,7 @@ def main():
         raise SystemExit("non-finite S3TTMc output")
     F = f_alg(t, N, R, nnz)
     achieved = F / (k_avg * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(kname, args.workload)
     log("S3TTMc %.3f ms/step (kernel %.3f ms), %.3e nnz/s, %.2f TFLOP/s alg"
         % (ms_step, k_avg, nnz / (ms_step * 1e-3), achieved))
 
@@ -212,7 +226,8 @@ def main():
         "roofline": {"bound": "mfma", "roof": "FP64 arithmetic (MI355X FP64 vector = "
                      "matrix peak)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "kernel": kname,
+                     "traffic": traffic, "traffic_unit": "bytes per launch",
+                     "traffic_source": traffic_src, "kernel": kname,
                      "kernel_ms": k_avg, "alg_flops": F,
                      "split_reduce_ms": r_ms / max(1, r_cnt)},
     }
