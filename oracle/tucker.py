@@ -22,8 +22,17 @@ def regularize(U, delta):
     return np.linalg.svd(U, full_matrices=False)[0]
 
 
+_UNPACK_MAPS = {}
+
+
 def unpack_rows(Y, R, k):
-    return np.stack([oracle.unpack(Y[i], R, k) for i in range(Y.shape[0])])
+    """Unpacked rows: column (j1..jk) of the full R^k row takes the packed
+    entry of the sorted multi-index (symtensor.hpp:96-116)."""
+    key = (R, k)
+    if key not in _UNPACK_MAPS:
+        P = oracle.symtens_size(R, k)
+        _UNPACK_MAPS[key] = oracle.unpack(np.arange(P, dtype=np.float64), R, k).astype(np.int64)
+    return Y[:, _UNPACK_MAPS[key]]
 
 
 def rel_residual(x, y):

@@ -279,7 +279,8 @@ void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *
 void orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   const u32 R = c.R;
   const u64 n = c.dim;
-  DevBuf g((u64)R * R * sizeof(double));
+  DevBuf &g = c.ws_rr;
+  g.reserve((u64)R * R * sizeof(double));
   atb(c, c.d_U.as<double>(), R, c.d_U.as<double>(), R, n, g.as<double>(), "gram_u");
   std::vector<double> G((size_t)R * R);
   SP_HIP(hipMemcpyAsync(G.data(), g.p, G.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -323,8 +324,10 @@ void dense_gemmtn(Ctx &c) {
 void dense_symgemm(Ctx &c) {
   const u64 n = c.dim;
   const u32 R = c.R;
-  DevBuf M((u64)R * c.P * sizeof(double));
-  DevBuf out(n * R * sizeof(double));
+  DevBuf &M = c.ws_rp;
+  DevBuf &out = c.ws_nr;
+  M.reserve((u64)R * c.P * sizeof(double));
+  out.reserve(n * R * sizeof(double));
   ScopedTimer t(c, "symgemm");
   scale_cols<<<grid_for((u64)R * c.P, 256), 256, 0, c.stream>>>(
       c.d_core.as<double>(), c.d_W.as<double>(), R, c.P, M.as<double>());
@@ -347,7 +350,8 @@ void dense_gram(Ctx &c) {
   u64 nk = std::max<u64>(1, std::min<u64>((n + 255) / 256, 1024 / ntiles + 1));
   const u64 rows_per = (n + nk - 1) / nk;
   nk = (n + rows_per - 1) / rows_per;
-  DevBuf dt(ntiles * sizeof(int2));
+  DevBuf &dt = c.ws_tiles;
+  dt.reserve(ntiles * sizeof(int2));
   SP_HIP(hipMemcpyAsync(dt.p, tiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice,
                         c.stream));
   c.d_scratch.reserve(nk * ntiles * GT * GT * sizeof(double));
@@ -370,27 +374,40 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   const u64 P = c.P, n = c.dim;
   const u32 R = c.R;
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds packed width");
-  DevBuf D(P * sizeof(double)), E(P * sizeof(double)), info(sizeof(rocblas_int));
+  // workspace: D (P), E (P), residual (1), then two int words (info, sweeps)
+  c.ws_eig.reserve((2 * P + 2) * sizeof(double) + 4 * sizeof(rocblas_int));
+  double *D = c.ws_eig.as<double>();
+  double *E = D + P;
+  double *resid = E + P;
+  rocblas_int *info = reinterpret_cast<rocblas_int *>(resid + 2);
   rocblas_handle h = handle_for(c);
-  {
+  if (c.eig_method == 1) {
+    ScopedTimer t(c, "syevj");
+    rocblas_status st = rocsolver_dsyevj(h, rocblas_esort_ascending, rocblas_evect_original,
+                                         rocblas_fill_upper, (rocblas_int)P,
+                                         c.d_G.as<double>(), (rocblas_int)P, 0.0, resid, 100,
+                                         info + 1, D, info);
+    if (st != rocblas_status_success) throw Error(SP_ERR_DEVICE, "rocsolver_dsyevj failed");
+  } else {
     ScopedTimer t(c, "syevd");
     rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
                                          (rocblas_int)P, c.d_G.as<double>(),
-                                         (rocblas_int)P, D.as<double>(), E.as<double>(),
-                                         info.as<rocblas_int>());
+                                         (rocblas_int)P, D, E, info);
     if (st != rocblas_status_success) throw Error(SP_ERR_DEVICE, "rocsolver_dsyevd failed");
   }
   std::vector<double> lam(P);
-  SP_HIP(hipMemcpyAsync(lam.data(), D.p, P * sizeof(double), hipMemcpyDeviceToHost,
-                        c.stream));
+  rocblas_int hinfo = 0;
+  SP_HIP(hipMemcpyAsync(lam.data(), D, P * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
+  if (hinfo != 0) throw Error(SP_ERR_NUMERIC, "eigensolver did not converge");
   const double lmax = std::max(std::fabs(lam[P - 1]), 1e-300);
   if (sigma)
     for (u32 r = 0; r < R; ++r) sigma[r] = std::sqrt(std::max(lam[P - 1 - r], 0.0));
-  DevBuf M((u64)R * P * sizeof(double));
+  DevBuf &M = c.ws_rp;
+  M.reserve((u64)R * P * sizeof(double));
   svd_map<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
-      c.d_G.as<double>(), D.as<double>(), c.d_W.as<double>(), P, R, lmax * 1e-30,
-      M.as<double>());
+      c.d_G.as<double>(), D, c.d_W.as<double>(), P, R, lmax * 1e-30, M.as<double>());
   SP_CHECK_LAUNCH();
   {
     ScopedTimer t(c, "y_times_v");
@@ -404,7 +421,10 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
 void dense_qr(Ctx &c) {
   const u64 n = c.dim;
   const u32 R = c.R;
-  DevBuf At(n * R * sizeof(double)), tau(R * sizeof(double));
+  DevBuf &At = c.ws_nr;
+  DevBuf &tau = c.ws_rr;
+  At.reserve(n * R * sizeof(double));
+  tau.reserve((u64)R * R * sizeof(double));
   transpose_nr<<<grid_for(n * R, 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
                                                            At.as<double>());
   SP_CHECK_LAUNCH();
@@ -426,9 +446,8 @@ void dense_regularize(Ctx &c, double delta) {
   const u64 n = c.dim;
   clip_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n, c.R, delta);
   SP_CHECK_LAUNCH();
-  DevBuf tmp;
-  orthonormalize_gram(c, tmp, false);
-  orthonormalize_gram(c, tmp, true);
+  orthonormalize_gram(c, c.ws_nr, false);
+  orthonormalize_gram(c, c.ws_nr, true);
 }
 
 double dense_core_norm(Ctx &c) {

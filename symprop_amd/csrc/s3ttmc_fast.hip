@@ -75,28 +75,28 @@ struct ProdTab {
   }
 };
 
-template <int R, int K>
-__device__ __forceinline__ double prod_entry(const double *p, const double *u, int J) {
-  constexpr ProdTab<R, K> T{};
-  double s = 0.0;
+template <int R, int K, class T>
+__device__ __forceinline__ T prod_entry(const T *p, const T *u, int J) {
+  constexpr ProdTab<R, K> tab{};
+  T s = 0;
 #pragma unroll
   for (int q = 0; q < K + 1; ++q)
-    if (q < T.n[J]) s = fma(p[T.src[J][q]], u[T.jj[J][q]], s);
+    if (q < tab.n[J]) s = fma(p[tab.src[J][q]], u[tab.jj[J][q]], s);
   return s;
 }
 
 // All coefficients of degree K+1 into registers.
-template <int R, int K>
-__device__ __forceinline__ void prod_full(const double *p, const double *u, double *out) {
+template <int R, int K, class T>
+__device__ __forceinline__ void prod_full(const T *p, const T *u, T *out) {
 #pragma unroll
   for (int J = 0; J < psz(R, K + 1); ++J) out[J] = prod_entry<R, K>(p, u, J);
 }
 
 // Quarter W of the degree-D polynomial of rows rows[0..D-1] (each R values),
-// scaled, into dst[J] for J in [lo, hi) of the quarter; zero padding up to
-// padded width (pad entries handled by the last quarter).
-template <int R, int D, int W>
-__device__ __forceinline__ void poly_quarter(const double *rows, double scale, double *dst,
+// scaled: store(J, value) for J in [lo, hi) of the quarter, and store(J, 0)
+// for the padding entries [P, padded) (handled by the last quarter).
+template <int R, int D, int W, class T, class Store>
+__device__ __forceinline__ void poly_quarter(const T *rows, T scale, Store &&store,
                                              int padded) {
   constexpr int P = psz(R, D);
   constexpr int Q = (P + 3) / 4;
@@ -104,19 +104,43 @@ __device__ __forceinline__ void poly_quarter(const double *rows, double scale, d
   constexpr int hi = (W + 1) * Q < P ? (W + 1) * Q : P;
   if constexpr (D == 1) {
 #pragma unroll
-    for (int J = lo; J < hi; ++J) dst[J] = rows[J] * scale;
+    for (int J = lo; J < hi; ++J) store(J, rows[J] * scale);
   } else if constexpr (D == 2) {
 #pragma unroll
-    for (int J = lo; J < hi; ++J) dst[J] = prod_entry<R, 1>(rows, rows + R, J) * scale;
+    for (int J = lo; J < hi; ++J) store(J, prod_entry<R, 1>(rows, rows + R, J) * scale);
   } else {
     static_assert(D == 3, "head/tail degree above 3 is not instantiated");
-    double p2[psz(R, 2)];
+    T p2[psz(R, 2)];
     prod_full<R, 1>(rows, rows + R, p2);
 #pragma unroll
-    for (int J = lo; J < hi; ++J) dst[J] = prod_entry<R, 2>(p2, rows + 2 * R, J) * scale;
+    for (int J = lo; J < hi; ++J) store(J, prod_entry<R, 2>(p2, rows + 2 * R, J) * scale);
   }
   if constexpr (W == 3) {
-    for (int J = P; J < padded; ++J) dst[J] = 0.0;
+    for (int J = P; J < padded; ++J) store(J, T(0));
+  }
+}
+
+// Quarter `wave` of the head (scaled by c) and of the tail (scaled by one).
+template <int R, int H, int TD, class T, class SA, class SB>
+__device__ __forceinline__ void produce_quarters(int wave, const T *rows, T c, T one, SA &&sa,
+                                                 SB &&sb, int padA, int padB) {
+  switch (wave) {
+    case 0:
+      poly_quarter<R, H, 0>(rows, c, sa, padA);
+      poly_quarter<R, TD, 0>(rows + H * R, one, sb, padB);
+      break;
+    case 1:
+      poly_quarter<R, H, 1>(rows, c, sa, padA);
+      poly_quarter<R, TD, 1>(rows + H * R, one, sb, padB);
+      break;
+    case 2:
+      poly_quarter<R, H, 2>(rows, c, sa, padA);
+      poly_quarter<R, TD, 2>(rows + H * R, one, sb, padB);
+      break;
+    default:
+      poly_quarter<R, H, 3>(rows, c, sa, padA);
+      poly_quarter<R, TD, 3>(rows + H * R, one, sb, padB);
+      break;
   }
 }
 
@@ -126,6 +150,8 @@ struct FastArgs {
   const u32 *tup;
   const double *coef;
   const double *U;
+  const float *coef32;
+  const float *U32;
   const u32 *fold_ptr;   // P_M + 1
   const u32 *fold_term;  // (h << 16) | t
   const double *fact;    // prod f_J! of the P_M outputs
@@ -159,23 +185,22 @@ __device__ __forceinline__ BatchV read_batch(const Batch *__restrict__ bs, u64 b
 // a branch, so the loads stay in flight until first use: lanes past the
 // batch read a valid owner (the batch's first, or owner 0 for an empty
 // batch) and get coefficient 0.
-template <int M>
-__device__ __forceinline__ void load_owner(const FastArgs &a, const BatchV &bt, int lane,
-                                           u32 (&tp)[M], double &cv, bool &in) {
+template <int M, class T>
+__device__ __forceinline__ void load_owner(const FastArgs &a, const T *coef, const BatchV &bt,
+                                           int lane, u32 (&tp)[M], T &cv, bool &in) {
   in = lane < bt.nb;
   const u64 o = in ? bt.o0 + lane : (bt.nb > 0 ? bt.o0 : 0);
   const u32 *t = a.tup + o * M;
 #pragma unroll
   for (int k = 0; k < M; ++k) tp[k] = t[k];
-  cv = a.coef[o];
+  cv = coef[o];
 }
 
-template <int M, int R>
-__device__ __forceinline__ void load_rows(const double *U, const u32 (&tp)[M],
-                                          double (&rows)[M * R]) {
+template <int M, int R, class T>
+__device__ __forceinline__ void load_rows(const T *U, const u32 (&tp)[M], T (&rows)[M * R]) {
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    const double *ur = U + (u64)tp[k] * R;
+    const T *ur = U + (u64)tp[k] * R;
 #pragma unroll
     for (int j = 0; j < R; ++j) rows[k * R + j] = ur[j];
   }
@@ -231,11 +256,11 @@ s3ttmc_mfma64(FastArgs a) {
     u32 tp0[M];
     double cv0;
     bool in0;
-    load_owner<M>(a, read_batch(a.batches, ch.b_begin), lane, tp0, cv0, in0);
+    load_owner<M>(a, a.coef, read_batch(a.batches, ch.b_begin), lane, tp0, cv0, in0);
     cur_c = in0 ? cv0 : 0.0;
     load_rows<M, R>(a.U, tp0, rows);
-    load_owner<M>(a, read_batch(a.batches, min(ch.b_begin + 1, blast)), lane, tpn, nxt_cv,
-                  nxt_in);
+    load_owner<M>(a, a.coef, read_batch(a.batches, min(ch.b_begin + 1, blast)), lane, tpn,
+                  nxt_cv, nxt_in);
   }
 
   d4 acc[TPW];
@@ -251,24 +276,17 @@ s3ttmc_mfma64(FastArgs a) {
       double *arow = sA + lane * LDA;
       double *brow = sB + lane * LDB;
       // padding lanes (lane >= nb) write zero rows: both scales are 0
-      const double c = cur_c;
       const double one = lane < nb ? 1.0 : 0.0;
-      switch (wave) {
-        case 0: poly_quarter<R, H, 0>(rows, c, arow, TH * 16);
-                poly_quarter<R, T, 0>(rows + H * R, one, brow, TT * 16); break;
-        case 1: poly_quarter<R, H, 1>(rows, c, arow, TH * 16);
-                poly_quarter<R, T, 1>(rows + H * R, one, brow, TT * 16); break;
-        case 2: poly_quarter<R, H, 2>(rows, c, arow, TH * 16);
-                poly_quarter<R, T, 2>(rows + H * R, one, brow, TT * 16); break;
-        default: poly_quarter<R, H, 3>(rows, c, arow, TH * 16);
-                 poly_quarter<R, T, 3>(rows + H * R, one, brow, TT * 16); break;
-      }
+      produce_quarters<R, H, T>(
+          wave, rows, cur_c, one, [&](int J, double v) { arow[J] = v; },
+          [&](int J, double v) { brow[J] = v; }, TH * 16, TT * 16);
     }
     __syncthreads();
     // ---- prefetch: U rows of batch b+1, tuple of batch b+2
     cur_c = nxt_in ? nxt_cv : 0.0;
     load_rows<M, R>(a.U, tpn, rows);
-    load_owner<M>(a, read_batch(a.batches, min(b + 2, blast)), lane, tpn, nxt_cv, nxt_in);
+    load_owner<M>(a, a.coef, read_batch(a.batches, min(b + 2, blast)), lane, tpn, nxt_cv,
+                  nxt_in);
     // ---- consumer: MFMA over the batch
     for (int ks = 0; ks < ksteps; ++ks) {
       const double *ar = sA + ks * 4 * LDA;
@@ -311,14 +329,129 @@ s3ttmc_mfma64(FastArgs a) {
   }
 }
 
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Mixed-precision form of s3ttmc_mfma64: FP32 U rows, coefficients and
+// polynomial products; A = c*head and B = tail rounded to BF16 (round to
+// nearest even) and multiplied on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation; the fold sums in FP64.  Operands are stored transposed
+// (coefficient-major, owner-minor) so that one 16-byte LDS read gives a
+// lane its 8 consecutive owners of the K dimension.  A batch of 64 owners is
+// two K-steps of 32.
+template <int M, int R, int H>
+__global__ void __launch_bounds__(256)
+s3ttmc_mfma_bf16(FastArgs a) {
+  constexpr int T = M - H;
+  constexpr int PH = psz(R, H), PT = psz(R, T), PM = psz(R, M);
+  constexpr int TH = (PH + 15) / 16, TT = (PT + 15) / 16;
+  constexpr int NTILE = TH * TT;
+  constexpr int TPW = (NTILE + 3) / 4;
+  constexpr int BO = 64;
+  constexpr int LDT = BO + 8;  // bf16 elements per coefficient row (16 B pad)
+  __shared__ __attribute__((aligned(16))) __bf16 sA[TH * 16 * LDT];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[TT * 16 * LDT];
+  __shared__ float sF[PH * PT];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int kr = lane >> 4, kc = lane & 15;
+  const Chunk ch = a.chunks[blockIdx.x];
+  if (ch.b_begin >= ch.b_end) return;
+
+  int aoff[TPW], boff[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int tile = wave + 4 * i < NTILE ? wave + 4 * i : NTILE - 1;
+    aoff[i] = ((tile / TT) * 16 + kc) * LDT + 8 * kr;
+    boff[i] = ((tile % TT) * 16 + kc) * LDT + 8 * kr;
+  }
+
+  const u64 blast = ch.b_end - 1;
+  u32 tpn[M];
+  float cur_c, nxt_cv;
+  bool nxt_in;
+  float rows[M * R];
+  {
+    u32 tp0[M];
+    float cv0;
+    bool in0;
+    load_owner<M>(a, a.coef32, read_batch(a.batches, ch.b_begin), lane, tp0, cv0, in0);
+    cur_c = in0 ? cv0 : 0.f;
+    load_rows<M, R>(a.U32, tp0, rows);
+    load_owner<M>(a, a.coef32, read_batch(a.batches, min(ch.b_begin + 1, blast)), lane, tpn,
+                  nxt_cv, nxt_in);
+  }
+
+  f4 acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+
+  for (u64 b = ch.b_begin; b < ch.b_end; ++b) {
+    const BatchV bt = read_batch(a.batches, b);
+    const int nb = bt.nb;
+    const int ksteps = (nb + 31) >> 5;
+    // ---- producer: every lane writes its owner column (zeros past nb)
+    if (nb > 0) {
+      const float one = lane < nb ? 1.f : 0.f;
+      produce_quarters<R, H, T>(
+          wave, rows, cur_c, one, [&](int J, float v) { sA[J * LDT + lane] = (__bf16)v; },
+          [&](int J, float v) { sB[J * LDT + lane] = (__bf16)v; }, TH * 16, TT * 16);
+    }
+    __syncthreads();
+    cur_c = nxt_in ? nxt_cv : 0.f;
+    load_rows<M, R>(a.U32, tpn, rows);
+    load_owner<M>(a, a.coef32, read_batch(a.batches, min(b + 2, blast)), lane, tpn, nxt_cv,
+                  nxt_in);
+    for (int ks = 0; ks < ksteps; ++ks) {
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        if (NTILE % 4 == 0 || wave + 4 * i < NTILE) {
+          const bf8 av = *reinterpret_cast<const bf8 *>(sA + aoff[i] + ks * 32);
+          const bf8 bv = *reinterpret_cast<const bf8 *>(sB + boff[i] + ks * 32);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+    if (!bt.last) continue;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int tile = wave + 4 * i;
+      if (tile < NTILE) {
+        const int th = tile / TT, tt = tile % TT;
+        const int t = tt * 16 + kc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int h = th * 16 + 4 * kr + q;
+          if (h < PH && t < PT) sF[h * PT + t] = acc[i][q];
+        }
+      }
+      acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    __syncthreads();
+    double *out = (ch.partial >= 0) ? a.partial + (u64)ch.partial * PM : a.Y + (u64)bt.row * PM;
+    for (int J = threadIdx.x; J < PM; J += 256) {
+      double s = 0.0;
+      for (u32 q = a.fold_ptr[J]; q < a.fold_ptr[J + 1]; ++q) {
+        const u32 e = a.fold_term[q];
+        s += (double)sF[(e >> 16) * PT + (e & 0xFFFFu)];
+      }
+      out[J] = s * a.fact[J];
+    }
+    __syncthreads();
+  }
+}
+
 using FastKernel = void (*)(FastArgs);
 
 struct FastEntry {
   int M, R, H;
-  FastKernel k;
+  FastKernel k64, kbf16;
 };
 
-#define SP_FAST(M, R, H) FastEntry{M, R, H, s3ttmc_mfma64<M, R, H>}
+#define SP_FAST(M, R, H) \
+  FastEntry { M, R, H, s3ttmc_mfma64<M, R, H>, s3ttmc_mfma_bf16<M, R, H> }
 
 const FastEntry kFast[] = {
     // order 3
@@ -381,20 +514,22 @@ void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32
 }
 
 // Returns the instantiated fast kernel for (M, R) or null; sets H.
-void *fast_kernel_for(int M, u32 R, int *H) {
+void *fast_kernel_for(int M, u32 R, int *H, void **kbf16) {
   for (const auto &e : kFast)
     if (e.M == M && (u32)e.R == R) {
       *H = e.H;
-      return (void *)e.k;
+      if (kbf16) *kbf16 = (void *)e.kbf16;
+      return (void *)e.k64;
     }
   return nullptr;
 }
 
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
                  const Batch *batches, const u32 *tup, const double *coef, const double *U,
-                 const u32 *fold_ptr, const u32 *fold_term, const double *fact, double *Y,
-                 double *partial) {
-  FastArgs a{chunks, batches, tup, coef, U, fold_ptr, fold_term, fact, Y, partial};
+                 const float *coef32, const float *U32, const u32 *fold_ptr,
+                 const u32 *fold_term, const double *fact, double *Y, double *partial) {
+  FastArgs a{chunks,   batches,   tup,  coef, U, coef32, U32, fold_ptr,
+             fold_term, fact,     Y,    partial};
   ((FastKernel)kernel)<<<grid, 256, 0, s>>>(a);
 }
 

@@ -155,6 +155,13 @@ struct Ctx {
   DevBuf d_G;       // f64 P x P
   DevBuf d_scratch;
   DevBuf d_partial;
+  // persistent workspaces of the dense Y-side ops (grown, never shrunk)
+  DevBuf ws_rp;    // R x P
+  DevBuf ws_nr;    // n x R
+  DevBuf ws_rr;    // R x R
+  DevBuf ws_eig;   // 2P + R doubles + info words
+  DevBuf ws_tiles; // Gram tile list
+  int eig_method = 0;  // 0: rocSOLVER syevd (divide and conquer), 1: syevj (Jacobi)
 
   // chunking of the work list for the current R
   std::vector<Chunk> h_chunks;
@@ -171,6 +178,7 @@ struct Ctx {
 
   // compile-time specialised MFMA kernel for (order-1, R), if instantiated
   void *fast_kernel = nullptr;
+  void *fast_kernel_bf16 = nullptr;
   int fast_H = 0;
   DevBuf d_fold_ptr, d_fold_term;
 
@@ -197,11 +205,11 @@ void radix_sort(K *keys, K *keys_tmp, u32 *vals, u32 *vals_tmp, u64 n,
 void s3ttmc_prepare(Ctx &c, u32 R);
 void s3ttmc_run(Ctx &c, int precision);
 void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
-void *fast_kernel_for(int M, u32 R, int *H);
+void *fast_kernel_for(int M, u32 R, int *H, void **kbf16);
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
                  const Batch *batches, const u32 *tup, const double *coef, const double *U,
-                 const u32 *fold_ptr, const u32 *fold_term, const double *fact, double *Y,
-                 double *partial);
+                 const float *coef32, const float *U32, const u32 *fold_ptr,
+                 const u32 *fold_term, const double *fact, double *Y, double *partial);
 
 // dense Y-side ops (dense.hip)
 void dense_gemmtn(Ctx &c);

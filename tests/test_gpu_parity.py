@@ -177,6 +177,24 @@ def test_s3ttmc_fp32(contact5):
     assert werr(Y32, Yo, 8, o) <= 2e-5
 
 
+@pytest.mark.parametrize("N,n,nnz,R,seed", [(4, 2000, 40000, 10, 1), (5, 300, 6000, 8, 2),
+                                            (6, 400, 5000, 8, 3), (7, 60, 1500, 6, 4)])
+def test_s3ttmc_bf16_mfma(N, n, nnz, R, seed):
+    # BF16 operands, FP32 accumulation: error well below 1e-2 and well above
+    # FP64 rounding (shows that the reduced-precision kernel ran)
+    inds, vals = sp.generate(sp.GEN_ZIPF, N, n, nnz, 1.1, 1, seed=seed)
+    U = np.linalg.qr(np.random.default_rng(seed).standard_normal((n, R)))[0]
+    say("s3ttmc bf16", N, n, len(vals), R)
+    t = sp.SpSymtensor(n, N, inds, vals)
+    Ybf = t.s3ttmc(U, precision=sp.PREC_BF16)
+    Y64 = t.s3ttmc(U)
+    e = werr(Ybf, Y64, R, N)
+    assert 1e-7 < e < 1e-2, e
+    t.timing(True)
+    t.s3ttmc_dev(sp.PREC_BF16)
+    assert t.timing_get("s3ttmc_mfma_bf16")[1] == 1
+
+
 def test_dense_ops_vs_numpy():
     rng = np.random.default_rng(21)
     N, n, R = 4, 50, 5
@@ -205,7 +223,9 @@ def test_dense_ops_vs_numpy():
     assert np.allclose(np.abs(Q.T @ Qref), np.eye(R), atol=1e-8)
 
 
-def test_leftsvd_subspace():
+@pytest.mark.parametrize("eig", ["syevd", "syevj"])
+def test_leftsvd_subspace(eig, monkeypatch):
+    monkeypatch.setenv("SP_EIG", eig)
     rng = np.random.default_rng(22)
     N, n, R = 4, 80, 4
     tt = np.unique(np.sort(rng.integers(1, n + 1, size=(4000, N)), axis=1), axis=0)

@@ -71,8 +71,10 @@ def test_mixed_precision_reaches_fp64_fit(prec, contact5):
     o, d, inds, vals = contact5
     t = sp.SpSymtensor(d, o, inds, vals)
     say("tucker mixed", prec)
-    r64 = t.tucker(alg=0, rank=8, max_iter=40, rel_tol=1e-8, delta=0.1, seed=0)
-    rmx = t.tucker(alg=0, rank=8, max_iter=40, rel_tol=1e-8, delta=0.1, seed=0,
+    # Config's default rel_tol (tucker.hpp:17); the contact-5 trajectory is slow and
+    # not monotone, so much tighter tolerances stop at noise-dependent iterations
+    r64 = t.tucker(alg=0, rank=8, max_iter=60, rel_tol=1e-6, delta=0.1, seed=0)
+    rmx = t.tucker(alg=0, rank=8, max_iter=60, rel_tol=1e-6, delta=0.1, seed=0,
                    precision=prec)
     assert rmx["fp64_sweeps"] >= 1
     e64, emx = r64["errors"][-1], rmx["errors"][-1]
