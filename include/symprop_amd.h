@@ -160,6 +160,34 @@ int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
                   int verbose, double *U_out, double *errors, uint64_t *iters,
                   uint64_t *fp64_sweeps, double *seconds);
 
+/* ---- Learned warm start.  No counterpart in the reference: the predicted
+ * factor enters at its initialisation hook (hosvd_init,
+ * src/sparse/spsymtensor.cpp:339-383) through sp_tucker_run's U0.
+ *
+ * Node features (dim x 64, FP32) of the context's tensor: log(1+degree),
+ * log(1+distinct co-occurrence partners) and a 62-dim sketch of hash-seeded
+ * Gaussian node vectors propagated `rounds` times through the owner lists. */
+int sp_node_features(sp_ctx *ctx, uint32_t rounds, uint64_t seed, float *out);
+
+/* MLP din -> dh -> dh -> dout (ReLU), FP32, trained with Adam on the mean
+ * squared row error; forward, backward and update run in HIP kernels.
+ * Parameters are flattened as W1[din][dh], b1, W2[dh][dh], b2, W3[dh][dout], b3. */
+typedef struct sp_mlp sp_mlp;
+int sp_mlp_create(uint32_t din, uint32_t dh, uint32_t dout, uint64_t seed, sp_mlp **out);
+int sp_mlp_destroy(sp_mlp *m);
+int sp_mlp_num_params(sp_mlp *m, uint64_t *n);
+int sp_mlp_get_params(sp_mlp *m, float *params);
+int sp_mlp_set_params(sp_mlp *m, const float *params);
+/* Loss and gradient of one batch (host buffers X rows x din, T rows x dout). */
+int sp_mlp_loss_grad(sp_mlp *m, const float *X, const float *T, uint64_t rows,
+                     double *loss, float *grad);
+/* `steps` minibatch Adam steps over seeded row permutations; losses (steps
+ * entries) and final_loss may be null. */
+int sp_mlp_train(sp_mlp *m, const float *X, const float *T, uint64_t rows,
+                 uint32_t steps, uint64_t batch, float lr, uint64_t seed,
+                 float *losses, double *final_loss);
+int sp_mlp_predict(sp_mlp *m, const float *X, uint64_t rows, float *out);
+
 /* Kernel timing with HIP events on the context's stream.  While enabled,
  * every kernel launch of the library is bracketed by events; sp_timing_get
  * sums the durations of launches whose name matches `name` exactly. */

@@ -34,6 +34,7 @@ GEN_PLANTED = 2
 u32p = ctypes.POINTER(ctypes.c_uint32)
 u64p = ctypes.POINTER(ctypes.c_uint64)
 f64p = ctypes.POINTER(ctypes.c_double)
+f32p = ctypes.POINTER(ctypes.c_float)
 vp = ctypes.c_void_p
 
 # name -> (restype, argtypes) of every exported C-ABI function
@@ -65,6 +66,18 @@ SIGNATURES = {
     "sp_regularize": (ctypes.c_int, [vp, ctypes.c_double]),
     "sp_core_norm": (ctypes.c_int, [vp, f64p]),
     "sp_ctx_info": (ctypes.c_int, [vp, u32p, u32p, u64p]),
+    "sp_node_features": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint64, f32p]),
+    "sp_mlp_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint64, ctypes.POINTER(vp)]),
+    "sp_mlp_destroy": (ctypes.c_int, [vp]),
+    "sp_mlp_num_params": (ctypes.c_int, [vp, u64p]),
+    "sp_mlp_get_params": (ctypes.c_int, [vp, f32p]),
+    "sp_mlp_set_params": (ctypes.c_int, [vp, f32p]),
+    "sp_mlp_loss_grad": (ctypes.c_int, [vp, f32p, f32p, ctypes.c_uint64, f64p, f32p]),
+    "sp_mlp_train": (ctypes.c_int, [vp, f32p, f32p, ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.c_uint64, ctypes.c_float, ctypes.c_uint64, f32p,
+                                    f64p]),
+    "sp_mlp_predict": (ctypes.c_int, [vp, f32p, ctypes.c_uint64, f32p]),
     "sp_tucker_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                      ctypes.c_char_p, ctypes.c_int, ctypes.c_double,

@@ -166,7 +166,7 @@ __global__ void build_worklist(const u32 *__restrict__ order_rec,  // owner pos
                                const double *__restrict__ own_val,
                                const K *__restrict__ leaf_keys, int m, int b,
                                u32 *__restrict__ tup, double *__restrict__ coef,
-                               float *__restrict__ coef32) {
+                               float *__restrict__ coef32, double *__restrict__ val) {
   const u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= no) return;
   const u32 o = order_rec[q];
@@ -203,6 +203,7 @@ __global__ void build_worklist(const u32 *__restrict__ order_rec,  // owner pos
   const double c = own_val[o] * (double)mult / (double)fact[m + 1];
   coef[q] = c;
   coef32[q] = (float)c;
+  val[q] = own_val[o];
 }
 
 __global__ void rem_keys(const u32 *__restrict__ rem, u64 no, u64 *__restrict__ key,
@@ -372,10 +373,11 @@ void build_worklist_t(Ctx &c, DevBuf &own_leaf) {
   c.w_tuple.alloc(no * m * sizeof(u32));
   c.w_coef.alloc(no * sizeof(double));
   c.w_coef32.alloc(no * sizeof(float));
+  c.w_val.alloc(no * sizeof(double));
   build_worklist<K><<<grid_for(no, BLK), BLK, 0, c.stream>>>(
       sid, no, own_leaf.as<u64>(), c.own_rem.as<u32>(), c.own_val.as<double>(),
       c.levels[c.depth - 1].keys.as<K>(), m, c.bits, c.w_tuple.as<u32>(),
-      c.w_coef.as<double>(), c.w_coef32.as<float>());
+      c.w_coef.as<double>(), c.w_coef32.as<float>(), c.w_val.as<double>());
   SP_CHECK_LAUNCH();
   c.w_segptr.alloc((c.dim + 1) * sizeof(u64));
   seg_bounds<<<grid_for((u64)c.dim + 1, BLK), BLK, 0, c.stream>>>(srem, no, c.dim,

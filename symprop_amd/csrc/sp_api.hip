@@ -327,6 +327,73 @@ int sp_core_norm(sp_ctx *ctx, double *out) {
   });
 }
 
+int sp_node_features(sp_ctx *ctx, uint32_t rounds, uint64_t seed, float *out) {
+  return guard(ctx, [&] {
+    need(ctx && out, SP_ERR_ARG, "null argument");
+    need(rounds <= 64, SP_ERR_ARG, "rounds must be <= 64");
+    node_features(ctx->c, (int)rounds, seed, out);
+  });
+}
+
+int sp_mlp_create(uint32_t din, uint32_t dh, uint32_t dout, uint64_t seed, sp_mlp **out) {
+  return guard(nullptr, [&] {
+    need(out != nullptr, SP_ERR_ARG, "out is null");
+    need(din >= 1 && dh >= 1 && dout >= 1 && din <= 4096 && dh <= 4096 && dout <= 4096,
+         SP_ERR_ARG, "layer widths must be in [1,4096]");
+    *out = reinterpret_cast<sp_mlp *>(mlp_create((int)din, (int)dh, (int)dout, seed));
+  });
+}
+
+int sp_mlp_destroy(sp_mlp *m) {
+  return guard(nullptr, [&] { mlp_destroy(reinterpret_cast<Mlp *>(m)); });
+}
+
+int sp_mlp_num_params(sp_mlp *m, uint64_t *n) {
+  return guard(nullptr, [&] {
+    need(m && n, SP_ERR_ARG, "null argument");
+    *n = (uint64_t)mlp_num_params(reinterpret_cast<Mlp *>(m));
+  });
+}
+
+int sp_mlp_get_params(sp_mlp *m, float *p) {
+  return guard(nullptr, [&] {
+    need(m && p, SP_ERR_ARG, "null argument");
+    mlp_get_params(reinterpret_cast<Mlp *>(m), p);
+  });
+}
+
+int sp_mlp_set_params(sp_mlp *m, const float *p) {
+  return guard(nullptr, [&] {
+    need(m && p, SP_ERR_ARG, "null argument");
+    mlp_set_params(reinterpret_cast<Mlp *>(m), p);
+  });
+}
+
+int sp_mlp_loss_grad(sp_mlp *m, const float *X, const float *T, uint64_t rows, double *loss,
+                     float *grad) {
+  return guard(nullptr, [&] {
+    need(m && X && T && loss && rows >= 1, SP_ERR_ARG, "null argument or empty batch");
+    *loss = mlp_loss_grad(reinterpret_cast<Mlp *>(m), X, T, (long)rows, grad);
+  });
+}
+
+int sp_mlp_train(sp_mlp *m, const float *X, const float *T, uint64_t rows, uint32_t steps,
+                 uint64_t batch, float lr, uint64_t seed, float *losses, double *final_loss) {
+  return guard(nullptr, [&] {
+    need(m && X && T && rows >= 1 && batch >= 1, SP_ERR_ARG, "null argument or empty batch");
+    const double L = mlp_train(reinterpret_cast<Mlp *>(m), X, T, (long)rows, (int)steps,
+                               (long)batch, lr, seed, losses);
+    if (final_loss) *final_loss = L;
+  });
+}
+
+int sp_mlp_predict(sp_mlp *m, const float *X, uint64_t rows, float *out) {
+  return guard(nullptr, [&] {
+    need(m && X && out && rows >= 1, SP_ERR_ARG, "null argument or empty batch");
+    mlp_predict(reinterpret_cast<Mlp *>(m), X, (long)rows, out);
+  });
+}
+
 int sp_timing_enable(sp_ctx *ctx, int on) {
   return guard(ctx, [&] {
     need(ctx, SP_ERR_ARG, "null context");

@@ -139,6 +139,7 @@ struct Ctx {
   DevBuf w_tuple;   // u32 [owners * m], 0-based rows of the leaf
   DevBuf w_coef;    // f64 [owners], v * multinomial / N!
   DevBuf w_coef32;  // f32 [owners]
+  DevBuf w_val;     // f64 [owners], the owner's tensor value
   DevBuf w_segptr;  // u64 [dim+1]
   std::vector<u64> h_segptr;
 
@@ -219,6 +220,19 @@ void dense_leftsvd_y(Ctx &c, double *sigma);
 void dense_qr(Ctx &c);
 void dense_regularize(Ctx &c, double delta);
 double dense_core_norm(Ctx &c);
+
+// learned warm start (learned.hip)
+struct Mlp;
+Mlp *mlp_create(int din, int dh, int dout, u64 seed);
+void mlp_destroy(Mlp *m);
+long mlp_num_params(Mlp *m);
+void mlp_get_params(Mlp *m, float *out);
+void mlp_set_params(Mlp *m, const float *in);
+double mlp_loss_grad(Mlp *m, const float *X, const float *T, long rows, float *grad);
+double mlp_train(Mlp *m, const float *X, const float *T, long rows, int steps, long batch,
+                 float lr, u64 seed, float *losses);
+void mlp_predict(Mlp *m, const float *X, long rows, float *out);
+void node_features(Ctx &c, int rounds, u64 seed, float *f);
 
 // error string of the C-ABI (sp_api.hip)
 void set_last_error(const std::string &msg);

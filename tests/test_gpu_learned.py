@@ -1,0 +1,78 @@
+"""GPU kernels of the learned warm start against the oracle (oracle/mlp.py)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp
+import symprop_amd as sp
+from symprop_amd import learned
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def test_features_match_oracle():
+    inds, vals = sp.generate(sp.GEN_PLANTED, 5, 400, 3000, 0.8, 8, seed=3)
+    say("features")
+    t = sp.SpSymtensor(400, 5, inds, vals)
+    F = learned.features(t, rounds=3, seed=11)
+    Fo = mlp.node_features(5, 400, inds, vals, rounds=3, seed=11)
+    assert F.shape == (400, 64)
+    assert np.allclose(F, Fo, rtol=1e-5, atol=1e-6)
+
+
+def test_loss_and_gradient_match_oracle():
+    rng = np.random.default_rng(0)
+    din, dh, dout, N = 64, 256, 8, 300
+    m = learned.MLP(din, dh, dout, seed=1)
+    p = m.params().astype(np.float64)
+    X = rng.standard_normal((N, din)).astype(np.float32)
+    T = rng.standard_normal((N, dout)).astype(np.float32)
+    say("mlp loss/grad")
+    L, g = m.loss_grad(X, T)
+    Lo, go = mlp.loss_grad(p, X.astype(np.float64), T.astype(np.float64), din, dh, dout)
+    assert abs(L - Lo) <= 1e-5 * abs(Lo)
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    # per parameter block
+    off = 0
+    for s in mlp.param_shapes(din, dh, dout):
+        k = int(np.prod(s))
+        gb, gob = g[off:off + k], go[off:off + k]
+        assert np.linalg.norm(gb - gob) <= 1e-4 * np.linalg.norm(gob) + 1e-9
+        off += k
+
+
+def test_adam_trajectory_matches_oracle():
+    rng = np.random.default_rng(1)
+    din, dh, dout, N = 64, 32, 4, 128
+    m = learned.MLP(din, dh, dout, seed=2)
+    p = m.params().astype(np.float64)
+    X = rng.standard_normal((N, din)).astype(np.float32)
+    T = rng.standard_normal((N, dout)).astype(np.float32)
+    say("adam trajectory")
+    losses = m.train(X, T, steps=5, batch=N, lr=1e-3, seed=0)   # full batch
+    mo = np.zeros_like(p)
+    vo = np.zeros_like(p)
+    ref = []
+    for s in range(1, 6):
+        L, g = mlp.loss_grad(p, X.astype(np.float64), T.astype(np.float64), din, dh, dout)
+        ref.append(L)
+        p, mo, vo = mlp.adam_step(p, g, mo, vo, s, 1e-3)
+    assert np.allclose(losses, ref, rtol=1e-4)
+    assert np.allclose(m.params(), p, rtol=1e-3, atol=1e-5)
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((2000, 64)).astype(np.float32)
+    W = rng.standard_normal((64, 8)).astype(np.float32) / 8
+    T = np.tanh(X @ W).astype(np.float32)
+    m = learned.MLP(64, 256, 8, seed=4)
+    say("training")
+    losses = m.train(X, T, steps=300, batch=512, lr=1e-3, seed=1)
+    assert losses[-20:].mean() < 0.2 * losses[:5].mean()
