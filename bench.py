@@ -162,6 +162,102 @@ def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0)
                       % (threads, len(errs))}
 
 
+def planted_basis(n, K):
+    E = np.zeros((n, K))
+    b = n // K
+    for c in range(K):
+        E[c * b:(c + 1) * b, c] = 1.0
+    return np.linalg.qr(E)[0]
+
+
+def learned_study(dim, nnz, pin, ntrain, ntest, iters, steps):
+    """Learned warm start on seeded order-5 planted-partition tensors
+    (8 communities, binary values, rank 8, TuckerHOOI with delta 0.5).
+
+    Training targets: the factors plain HOOI converges to on `ntrain` seeds,
+    started from the hypergraph sketch (learned.spectral_start).  Held-out
+    seeds run HOOI from random, HOEVD, sketch and learned starts until the
+    subspace moves by less than 1e-6 (sin of the largest principal angle) or
+    `iters` iterations; each start reports iterations, converged fit error,
+    angle to the planted communities and time to converge including the cost
+    of producing the start."""
+    from symprop_amd import learned
+    N, K, R, delta = 5, 8, 8, 0.5
+    E = planted_basis(dim, K)
+
+    def tensor(seed):
+        inds, vals = sp.generate(sp.GEN_PLANTED, N, dim, nnz, pin, K, seed=seed)
+        t = sp.SpSymtensor(dim, N, inds, vals)
+        t.sync()
+        return t
+
+    feats, facs, teacher_its = [], [], []
+    t0 = time.perf_counter()
+    for s in range(1, ntrain + 1):
+        t = tensor(s)
+        F = learned.features(t)
+        Us, _ = learned.hooi_trace(t, learned.spectral_start(F, R), iters, delta,
+                                   stop_tol=1e-6)
+        teacher_its.append(learned.converged_at(Us))
+        feats.append(F)
+        facs.append(Us[-1])
+        del t
+    t_teacher = time.perf_counter() - t0
+    model = learned.WarmStartModel(R, seed=0)
+    t0 = time.perf_counter()
+    losses = model.fit(feats, facs, steps=steps, batch=4096, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    log("learned: %d teacher runs %.1f s, fit %.1f s, loss %.4f -> %.4f"
+        % (ntrain, t_teacher, t_fit, float(losses[:5].mean()), float(losses[-10:].mean())))
+
+    starts = ("random", "hoevd", "sketch", "learned")
+    rows = {k: [] for k in starts}
+    for s in range(10001, 10001 + ntest):
+        t = tensor(s)
+        for k in starts:
+            t0 = time.perf_counter()
+            if k == "random":
+                U0 = np.random.default_rng(s).standard_normal((dim, R))
+            elif k == "hoevd":
+                U0, _ = t.hoevd_init(R)
+            else:
+                F = learned.features(t)
+                U0 = learned.spectral_start(F, R) if k == "sketch" else model.predict(F)
+            t_init = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            Us, errs = learned.hooi_trace(t, U0, iters, delta, stop_tol=1e-6)
+            t_run = time.perf_counter() - t0
+            kc = learned.converged_at(Us)
+            rows[k].append({"seed": s, "converged_at": kc, "iters_run": len(Us),
+                            "final_error": float(errs[-1]),
+                            "sin_to_planted": learned.sin_angle(Us[-1], E),
+                            "init_s": t_init, "seconds": t_init + t_run})
+        log("learned seed %d: " % s + ", ".join(
+            "%s %s it err %.9f" % (k, rows[k][-1]["converged_at"], rows[k][-1]["final_error"])
+            for k in starts))
+        del t
+
+    def summary(rs):
+        conv = [r["converged_at"] for r in rs if r["converged_at"] is not None]
+        return {"converged": "%d/%d" % (len(conv), len(rs)),
+                "median_iters_to_1e-6": float(np.median(conv)) if conv else None,
+                "mean_final_error": float(np.mean([r["final_error"] for r in rs])),
+                "mean_sin_to_planted": float(np.mean([r["sin_to_planted"] for r in rs])),
+                "median_seconds": float(np.median([r["seconds"] for r in rs])),
+                "runs": rs}
+
+    return {"config": {"order": N, "dim": dim, "nnz": nnz, "p_in": pin, "communities": K,
+                       "rank": R, "delta": delta, "max_iters": iters,
+                       "train_tensors": ntrain, "held_out": ntest,
+                       "criterion": "sin(U_k, U_k-1) < 1e-6"},
+            "train": {"teacher": "HOOI from the sketch start",
+                      "teacher_converged_at": teacher_its, "teacher_s": t_teacher,
+                      "fit_s": t_fit, "steps": steps,
+                      "loss_first": float(losses[:5].mean()),
+                      "loss_last": float(losses[-10:].mean())},
+            "held_out": {k: summary(rows[k]) for k in starts}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +266,15 @@ def main():
     ap.add_argument("--workload", default="zipf4")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hooi", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-only", action="store_true")
+    ap.add_argument("--learned-dim", type=int, default=20000)
+    ap.add_argument("--learned-nnz", type=int, default=2_000_000)
+    ap.add_argument("--learned-pin", type=float, default=0.8)
+    ap.add_argument("--learned-train", type=int, default=8)
+    ap.add_argument("--learned-test", type=int, default=4)
+    ap.add_argument("--learned-iters", type=int, default=40)
+    ap.add_argument("--learned-steps", type=int, default=800)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(reference for HBM counter scaling) and exit")
@@ -188,6 +293,11 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    learned_args = (args.learned_dim, args.learned_nnz, args.learned_pin, args.learned_train,
+                    args.learned_test, args.learned_iters, args.learned_steps)
+    if args.learned_only:
+        print(json.dumps({"learned_warm_start": learned_study(*learned_args)}), flush=True)
+        return
 
     w = workload(args.workload)
     N, n, R = w["order"], w["dim"], w["rank"]
@@ -283,6 +393,11 @@ def main():
                 if "final_error" in v:
                     v["rel_fit_error_vs_fp64"] = abs(v["final_error"] - e64) / e64
         out["hooi_time_to_fit"] = runs
+
+    if not args.no_learned:
+        del t
+        log("learned warm start study")
+        out["learned_warm_start"] = learned_study(*learned_args)
 
     if not args.no_cpu:
         log("cpu baseline (reference build on host cores)")

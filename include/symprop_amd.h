@@ -160,6 +160,21 @@ int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
                   int verbose, double *U_out, double *errors, uint64_t *iters,
                   uint64_t *fp64_sweeps, double *seconds);
 
+/* ---- HOEVD initialisation (replaces hosvd_init's "hoevd" branch,
+ * src/sparse/spsymtensor.cpp:363-382, with sparse_syrk 385-409 and
+ * convert_to_coo 411-439).
+ *
+ * sp_hoevd_gram: A = X_(1) X_(1)^T (dim x dim, row-major FP64) of the built
+ * tensor, exact for every order and dimension (the reference's base-n column
+ * key overflows once dim^(order-1) >= 2^64).
+ * sp_hoevd_init: eigenvectors of the `R` largest eigenvalues of A, ascending
+ * (as matrix::syevd's last R columns, src/dense/matrixops.cpp:77-100); the
+ * result becomes the context's factor and is copied to U (dim x R, row-major)
+ * and the eigenvalues to eigvals (R) when those are not NULL.  Dense
+ * eigensolver: dim^2 must stay below 2^31. */
+int sp_hoevd_gram(sp_ctx *ctx, double *A);
+int sp_hoevd_init(sp_ctx *ctx, uint32_t R, double *U, double *eigvals);
+
 /* ---- Learned warm start.  No counterpart in the reference: the predicted
  * factor enters at its initialisation hook (hosvd_init,
  * src/sparse/spsymtensor.cpp:339-383) through sp_tucker_run's U0.

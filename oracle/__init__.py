@@ -72,6 +72,8 @@ def lib():
         L.orc_s3ttmc_dense.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                        ctypes.c_uint64, u32p, f64p, f64p,
                                        ctypes.c_uint32, f64p]
+        L.orc_hoevd_gram.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                     u32p, f64p, f64p]
         L.orc_random_factor.argtypes = [ctypes.c_uint64, ctypes.c_uint64, f64p]
         L.orc_uniform_factor.argtypes = [ctypes.c_uint64, ctypes.c_uint64, f64p]
         L.orc_load_tns.restype = ctypes.c_longlong
@@ -189,6 +191,22 @@ def s3ttmc_dense(order, dim, inds, vals, U, R):
     lib().orc_s3ttmc_dense(order, dim, len(vals), _p(inds, u32p), _p(vals, f64p),
                            _p(U, f64p), R, _p(Y, f64p))
     return Y
+
+
+def hoevd_gram(order, dim, inds, vals):
+    """X_(1) X_(1)^T as the reference's HOEVD builds it (dim x dim)."""
+    inds = np.ascontiguousarray(inds, dtype=np.uint32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    A = np.zeros((dim, dim))
+    lib().orc_hoevd_gram(order, dim, len(vals), _p(inds, u32p), _p(vals, f64p), _p(A, f64p))
+    return A
+
+
+def hoevd_factor(A, R):
+    """Eigenvectors of the R largest eigenvalues of A, ascending
+    (src/dense/matrixops.cpp:77-100), and those eigenvalues."""
+    w, V = np.linalg.eigh(A)
+    return V[:, -R:], w[-R:]
 
 
 def random_factor(seed, n, R):

@@ -16,6 +16,7 @@
 //   S3TTMc (opt5) ......... src/compressed_dtree.cpp:421-518
 //   S3TTMc driver ......... src/sparse/spsymtensor.cpp:30-110
 //   norm .................. src/sparse/spsymtensor.cpp:114-138
+//   HOEVD Gram ............ src/sparse/spsymtensor.cpp:385-439
 //   loader ................ src/utils/loader.cpp:8-36
 //   random init ........... src/tucker.cpp:25-36
 //
@@ -488,6 +489,50 @@ void orc_s3ttmc_dense(u32 order, u32 dim, u64 nnz, const u32 *inds,
     if (q < 0) break;
     idx[q]++;
     for (int t = q + 1; t < k; ++t) idx[t] = idx[q];
+  }
+}
+
+// HOEVD's A = X_(1) X_(1)^T as spsymtensor.cpp:385-439 builds it: every
+// distinct permutation of every nonzero is a COO entry (column = the last N-1
+// positions, row = the first), entries are sorted by (column, row), and each
+// column adds v1*v2 for every pair k <= l of its entries to A[row1][row2] and,
+// off the diagonal, to A[row2][row1].  The column is kept as the index tuple
+// itself rather than a base-n integer, so it cannot overflow.  Cost
+// nnz * N! entries; tiny inputs only.  A (dim x dim) is overwritten.
+void orc_hoevd_gram(u32 order, u32 dim, u64 nnz, const u32 *inds, const double *vals,
+                    double *A) {
+  const int N = (int)order;
+  struct Entry {
+    std::vector<u32> col;
+    u32 row;
+    double v;
+  };
+  std::vector<Entry> coo;
+  std::vector<u32> perm(N);
+  for (u64 i = 0; i < nnz; ++i) {
+    for (int j = 0; j < N; ++j) perm[j] = inds[i * N + j] - 1;
+    std::sort(perm.begin(), perm.end());
+    do {
+      coo.push_back({std::vector<u32>(perm.begin() + 1, perm.end()), perm[0], vals[i]});
+    } while (std::next_permutation(perm.begin(), perm.end()));
+  }
+  std::sort(coo.begin(), coo.end(), [](const Entry &a, const Entry &b) {
+    if (a.col != b.col) return a.col < b.col;
+    return a.row < b.row;
+  });
+  std::fill(A, A + (u64)dim * dim, 0.0);
+  u64 i = 0;
+  while (i < coo.size()) {
+    u64 j = i + 1;
+    while (j < coo.size() && coo[j].col == coo[i].col) ++j;
+    for (u64 k = i; k < j; ++k)
+      for (u64 l = k; l < j; ++l) {
+        const u32 r1 = coo[k].row, r2 = coo[l].row;
+        const double p = coo[k].v * coo[l].v;
+        A[(u64)r1 * dim + r2] += p;
+        if (r1 != r2) A[(u64)r2 * dim + r1] += p;
+      }
+    i = j;
   }
 }
 

@@ -66,6 +66,8 @@ SIGNATURES = {
     "sp_regularize": (ctypes.c_int, [vp, ctypes.c_double]),
     "sp_core_norm": (ctypes.c_int, [vp, f64p]),
     "sp_ctx_info": (ctypes.c_int, [vp, u32p, u32p, u64p]),
+    "sp_hoevd_gram": (ctypes.c_int, [vp, f64p]),
+    "sp_hoevd_init": (ctypes.c_int, [vp, ctypes.c_uint32, f64p, f64p]),
     "sp_node_features": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint64, f32p]),
     "sp_mlp_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(vp)]),
@@ -275,6 +277,21 @@ class SpSymtensor:
         v = ctypes.c_double()
         _check(lib().sp_core_norm(self.h, ctypes.byref(v)))
         return v.value
+
+    def hoevd_gram(self):
+        """A = X_(1) X_(1)^T (dim x dim)."""
+        A = np.zeros((self.dim, self.dim))
+        _check(lib().sp_hoevd_gram(self.h, _p(A, f64p)))
+        return A
+
+    def hoevd_init(self, rank):
+        """HOEVD factor (top-`rank` eigenvectors of X_(1) X_(1)^T, ascending);
+        it also becomes the current factor.  Returns (U, eigenvalues)."""
+        U = np.zeros((self.dim, rank))
+        w = np.zeros(rank)
+        _check(lib().sp_hoevd_init(self.h, rank, _p(U, f64p), _p(w, f64p)))
+        self.R = rank
+        return U, w
 
     # decomposition -------------------------------------------------------
     def tucker(self, alg=0, rank=2, max_iter=10, rel_tol=1e-6, delta=0.5, seed=0,

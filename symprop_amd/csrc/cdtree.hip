@@ -166,11 +166,14 @@ __global__ void build_worklist(const u32 *__restrict__ order_rec,  // owner pos
                                const double *__restrict__ own_val,
                                const K *__restrict__ leaf_keys, int m, int b,
                                u32 *__restrict__ tup, double *__restrict__ coef,
-                               float *__restrict__ coef32, double *__restrict__ val) {
+                               float *__restrict__ coef32, double *__restrict__ val,
+                               u32 *__restrict__ leaf) {
   const u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= no) return;
   const u32 o = order_rec[q];
-  const K key = leaf_keys[own_leaf[o]];
+  const u64 lid = own_leaf[o];
+  leaf[q] = (u32)lid;
+  const K key = leaf_keys[lid];
   const u32 rem = own_rem[o];
   u32 t[16];
   const K mask = (K)((1ull << b) - 1);
@@ -374,10 +377,14 @@ void build_worklist_t(Ctx &c, DevBuf &own_leaf) {
   c.w_coef.alloc(no * sizeof(double));
   c.w_coef32.alloc(no * sizeof(float));
   c.w_val.alloc(no * sizeof(double));
+  c.w_leaf.alloc(no * sizeof(u32));
+  if (c.levels[c.depth - 1].size >= (1ull << 32))
+    throw Error(SP_ERR_UNSUPPORTED, "more than 2^32 leaves");
   build_worklist<K><<<grid_for(no, BLK), BLK, 0, c.stream>>>(
       sid, no, own_leaf.as<u64>(), c.own_rem.as<u32>(), c.own_val.as<double>(),
       c.levels[c.depth - 1].keys.as<K>(), m, c.bits, c.w_tuple.as<u32>(),
-      c.w_coef.as<double>(), c.w_coef32.as<float>(), c.w_val.as<double>());
+      c.w_coef.as<double>(), c.w_coef32.as<float>(), c.w_val.as<double>(),
+      c.w_leaf.as<u32>());
   SP_CHECK_LAUNCH();
   c.w_segptr.alloc((c.dim + 1) * sizeof(u64));
   seg_bounds<<<grid_for((u64)c.dim + 1, BLK), BLK, 0, c.stream>>>(srem, no, c.dim,

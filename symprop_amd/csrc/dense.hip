@@ -25,6 +25,10 @@ namespace spamd {
 
 namespace {
 
+// Relative eigenvalue floor of a Gram matrix below which a direction counts
+// as numerically absent (singular value ratio ~3e-6).
+constexpr double kGramRankTol = 1e-11;
+
 constexpr int RB = 16;  // rank columns per register block
 
 // partial[b][r][p] = sum over rows of block b of A[i][r] * B[i][p]
@@ -176,6 +180,22 @@ __global__ void transpose_nr(const double *__restrict__ in, u64 rows, u64 cols,
   out[j * rows + i] = in[t];
 }
 
+// Columns from..R-1 of U (n x R) <- fixed pseudo-random values in [-1, 1)
+// (splitmix64 of the entry's position), the start of an orthonormal
+// completion when the span to orthonormalise has fewer than R dimensions.
+__global__ void fill_columns(double *__restrict__ U, u64 n, u32 R, u32 from) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 w = R - from;
+  if (t >= n * w) return;
+  const u64 i = t / w;
+  const u32 j = from + (u32)(t % w);
+  u64 z = i * (u64)R + j + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  U[i * R + j] = (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+
 __global__ void clip_rows(double *__restrict__ U, u64 n, u32 R, double delta) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -276,7 +296,11 @@ void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *
 // U <- U V Lambda^{-1/2} from the R x R Gram U^T U: the left singular
 // vectors of U in descending order (first pass), or U <- U (U^T U)^{-1/2}
 // (symmetric pass, which keeps the column order of a nearly orthonormal U).
-void orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
+// Orthonormalises the columns of U through the eigen-decomposition of its
+// R x R Gram (U <- U V w^{-1/2}, or the symmetric U V w^{-1/2} V^T).
+// Directions whose Gram eigenvalue is below kGramRankTol of the largest are
+// dropped and come out as zero columns; returns the number kept.
+u32 orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   const u32 R = c.R;
   const u64 n = c.dim;
   DevBuf &g = c.ws_rr;
@@ -288,17 +312,19 @@ void orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   SP_HIP(hipStreamSynchronize(c.stream));
   std::vector<double> w, V;
   jacobi_eig(G, (int)R, w, V);
+  u32 good = 0;
+  while (good < R && std::isfinite(w[good]) && w[good] > 0.0 && w[good] > w[0] * kGramRankTol)
+    ++good;
   // M[r][s] (R x R, "r" = output column) = V[s][r] / sqrt(w[r])
-  std::vector<double> M((size_t)R * R);
-  const double floor_ = std::max(w[0], 1e-300) * 1e-30;
+  std::vector<double> M((size_t)R * R, 0.0);
   for (u32 r = 0; r < R; ++r)
     for (u32 s = 0; s < R; ++s) {
       if (!symmetric) {
-        M[(size_t)r * R + s] = V[(size_t)s * R + r] / std::sqrt(std::max(w[r], floor_));
+        if (r < good) M[(size_t)r * R + s] = V[(size_t)s * R + r] / std::sqrt(w[r]);
       } else {
         double v = 0.0;
-        for (u32 q = 0; q < R; ++q)
-          v += V[(size_t)s * R + q] * V[(size_t)r * R + q] / std::sqrt(std::max(w[q], floor_));
+        for (u32 q = 0; q < good; ++q)
+          v += V[(size_t)s * R + q] * V[(size_t)r * R + q] / std::sqrt(w[q]);
         M[(size_t)r * R + s] = v;
       }
     }
@@ -311,9 +337,26 @@ void orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
                         c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
+  return good;
 }
 
 }  // namespace
+
+// U's first `good` columns are orthonormal; replace the rest by an
+// orthonormal completion (fixed pseudo-random columns, then thin QR, which
+// keeps the span of the leading columns).  LAPACK's SVD likewise returns
+// some orthonormal basis for the directions with zero singular value.
+void complete_basis(Ctx &c, u32 good) {
+  if (good >= c.R) return;
+  const u64 n = c.dim;
+  if (c.R > n) throw Error(SP_ERR_ARG, "rank exceeds dimension");
+  fill_columns<<<grid_for(n * (c.R - good), 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n,
+                                                                     c.R, good);
+  SP_CHECK_LAUNCH();
+  dense_qr(c);
+}
+
+rocblas_handle blas_handle(Ctx &c) { return handle_for(c); }
 
 void dense_gemmtn(Ctx &c) {
   c.d_core.reserve((u64)c.R * c.P * sizeof(double));
@@ -404,6 +447,14 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   const double lmax = std::max(std::fabs(lam[P - 1]), 1e-300);
   if (sigma)
     for (u32 r = 0; r < R; ++r) sigma[r] = std::sqrt(std::max(lam[P - 1 - r], 0.0));
+  // The Gram's eigenvalues carry absolute errors of about eps * lambda_max,
+  // so directions with lambda below kGramRankTol * lambda_max (singular value
+  // below ~3e-6 of the largest), and all of them for a zero Y, have no
+  // reliable left vector; they get an orthonormal completion
+  u32 good = 0;
+  while (good < R && lam[P - 1 - good] > 0.0 &&
+         lam[P - 1 - good] > lam[P - 1] * kGramRankTol)
+    ++good;
   DevBuf &M = c.ws_rp;
   M.reserve((u64)R * P * sizeof(double));
   svd_map<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
@@ -415,6 +466,7 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
                                                      M.as<double>(), R, c.d_U.as<double>());
     SP_CHECK_LAUNCH();
   }
+  complete_basis(c, good);
   SP_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -446,7 +498,7 @@ void dense_regularize(Ctx &c, double delta) {
   const u64 n = c.dim;
   clip_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n, c.R, delta);
   SP_CHECK_LAUNCH();
-  orthonormalize_gram(c, c.ws_nr, false);
+  complete_basis(c, orthonormalize_gram(c, c.ws_nr, false));
   orthonormalize_gram(c, c.ws_nr, true);
 }
 

@@ -327,6 +327,35 @@ int sp_core_norm(sp_ctx *ctx, double *out) {
   });
 }
 
+int sp_hoevd_gram(sp_ctx *ctx, double *A) {
+  return guard(ctx, [&] {
+    need(ctx && A, SP_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    need(c.built, SP_ERR_STATE, "tree not built");
+    const u64 n = c.dim;
+    DevBuf d(n * n * sizeof(double));
+    hoevd_gram(c, d.as<double>());
+    SP_HIP(hipMemcpyAsync(A, d.p, n * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int sp_hoevd_init(sp_ctx *ctx, uint32_t R, double *U, double *eigvals) {
+  std::vector<double> Uh;
+  const int rc = guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    need(c.built, SP_ERR_STATE, "tree not built");
+    need(R >= 1 && R <= 255, SP_ERR_UNSUPPORTED, "rank must be in [1,255]");
+    need(R <= c.dim, SP_ERR_ARG, "rank exceeds dimension");
+    Uh.resize((u64)c.dim * R);
+    hoevd_init(c, R, Uh.data(), eigvals);
+    if (U) std::copy(Uh.begin(), Uh.end(), U);
+  });
+  if (rc) return rc;
+  return sp_factor_upload(ctx, Uh.data(), R);
+}
+
 int sp_node_features(sp_ctx *ctx, uint32_t rounds, uint64_t seed, float *out) {
   return guard(ctx, [&] {
     need(ctx && out, SP_ERR_ARG, "null argument");

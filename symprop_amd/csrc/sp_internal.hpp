@@ -140,6 +140,7 @@ struct Ctx {
   DevBuf w_coef;    // f64 [owners], v * multinomial / N!
   DevBuf w_coef32;  // f32 [owners]
   DevBuf w_val;     // f64 [owners], the owner's tensor value
+  DevBuf w_leaf;    // u32 [owners], leaf id (index into the last level)
   DevBuf w_segptr;  // u64 [dim+1]
   std::vector<u64> h_segptr;
 
@@ -220,6 +221,10 @@ void dense_leftsvd_y(Ctx &c, double *sigma);
 void dense_qr(Ctx &c);
 void dense_regularize(Ctx &c, double delta);
 double dense_core_norm(Ctx &c);
+
+// HOEVD initial factor (hoevd.hip)
+void hoevd_gram(Ctx &c, double *A);
+void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out);
 
 // learned warm start (learned.hip)
 struct Mlp;

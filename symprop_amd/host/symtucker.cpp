@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <iomanip>
 #include <iostream>
 #include <mutex>
@@ -304,8 +305,21 @@ void SpSymtensor::init_factor(std::vector<real_t> &U, const Config &c) const {
   }
 }
 
-void SpSymtensor::hoevd_init(std::vector<real_t> &, const Config &) const {
-  throw std::runtime_error("hoevd initialisation is not available in this build");
+// spsymtensor.cpp:363-382.  The reference also prints the whole dim x dim
+// matrix A to stdout (374-379); that debug dump is reproduced only when the
+// environment sets SP_HOEVD_PRINT=1.
+void SpSymtensor::hoevd_init(std::vector<real_t> &U, const Config &c) const {
+  const char *pr = std::getenv("SP_HOEVD_PRINT");
+  if (pr && std::string(pr) == "1") {
+    std::vector<real_t> A((size_t)_dim * _dim);
+    ck(sp_hoevd_gram(_ctx, A.data()));
+    for (size_t i = 0; i < _dim; i++) {
+      for (size_t j = 0; j < _dim; j++) std::cout << A[i * _dim + j] << ",";
+      std::cout << "\n";
+    }
+  }
+  U.resize((size_t)_dim * c.rank);
+  ck(sp_hoevd_init(_ctx, c.rank, U.data(), nullptr));
 }
 
 // spsymtensor.cpp:201-252

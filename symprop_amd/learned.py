@@ -120,22 +120,50 @@ class WarmStartModel:
         return self.mlp.predict(Xs).astype(np.float64) / np.sqrt(feats.shape[0])
 
 
-def hooi_trace(t: SpSymtensor, U0, iters, delta=0.5):
-    """TuckerHOOI's iteration from U0 (src/sparse/spsymtensor.cpp:221-241),
-    returning the factor after every iteration and the reported errors."""
-    R = U0.shape[1]
+def spectral_start(feats, rank):
+    """Non-learned start from the same features: the top-`rank` principal
+    directions (left singular vectors) of the centred hypergraph sketch."""
+    X = np.asarray(feats, dtype=np.float64)[:, 2:]
+    X = X - X.mean(axis=0)
+    return np.linalg.svd(X, full_matrices=False)[0][:, :rank]
+
+
+def hooi_trace(t: SpSymtensor, U0, iters, delta=0.5, alg="hooi", stop_tol=None):
+    """TuckerHOOI's iteration (src/sparse/spsymtensor.cpp:221-241) or
+    TuckerHOQRI's (src/sparse/spsymtensor.cpp:267-281) from U0, returning the
+    factor after every iteration and the reported errors.  With `stop_tol`,
+    stops after the first iteration whose subspace moved by less than it
+    (sin of the largest principal angle to the previous factor)."""
+    if alg not in ("hooi", "hoqri"):
+        raise ValueError("alg must be 'hooi' or 'hoqri'")
     t.set_factor(U0)
     norm = t.norm()
     Us, errs = [], []
     for _ in range(iters):
         t.regularize(delta)
         t.s3ttmc_dev()
-        t.leftsvd_y()
-        t.gemmtn()
+        if alg == "hooi":
+            t.leftsvd_y()
+            t.gemmtn()
+        else:
+            t.gemmtn()
+            t.symgemm()
+            t.qr()
         cn = t.core_norm()
         errs.append(np.sqrt(max(norm * norm - cn * cn, 0.0)) / norm)
         Us.append(t.factor())
+        if stop_tol is not None and len(Us) > 1 and sin_angle(Us[-1], Us[-2]) < stop_tol:
+            break
     return Us, np.array(errs)
+
+
+def converged_at(Us, tol=1e-6):
+    """First iteration (1-based) whose factor moved by less than `tol` from
+    the previous one; None if none did."""
+    for k in range(1, len(Us)):
+        if sin_angle(Us[k], Us[k - 1]) < tol:
+            return k + 1
+    return None
 
 
 def sin_angle(U1, U2):

@@ -67,6 +67,62 @@ def test_adam_trajectory_matches_oracle():
     assert np.allclose(m.params(), p, rtol=1e-3, atol=1e-5)
 
 
+def planted_basis(n, K):
+    E = np.zeros((n, K))
+    b = n // K
+    for c in range(K):
+        E[c * b:(c + 1) * b, c] = 1.0
+    return np.linalg.qr(E)[0]
+
+
+def test_learned_start_gain_over_random_and_hoevd():
+    """Order-5 planted 8-community tensors (n=1000, 500k binary tuples,
+    p_in=0.9), rank 8, HOOI with delta=0.5.  The network is trained on the
+    factors plain HOOI converges to on 4 training seeds (started from the
+    hypergraph sketch), then judged on 2 held-out seeds against random and
+    HOEVD starts: iterations until the subspace moves by < 1e-6, converged
+    fit, and angle to the planted communities."""
+    N, n, nnz, K, R, delta, iters = 5, 1000, 500_000, 8, 8, 0.5, 30
+    E = planted_basis(n, K)
+
+    def tensor(seed):
+        inds, vals = sp.generate(sp.GEN_PLANTED, N, n, nnz, 0.9, K, seed=seed)
+        return sp.SpSymtensor(n, N, inds, vals)
+
+    feats, facs = [], []
+    for s in range(1, 5):
+        say("teacher seed", s)
+        t = tensor(s)
+        F = learned.features(t)
+        Us, _ = learned.hooi_trace(t, learned.spectral_start(F, R), iters, delta,
+                                   stop_tol=1e-6)
+        assert learned.converged_at(Us) is not None
+        feats.append(F)
+        facs.append(Us[-1])
+    model = learned.WarmStartModel(R, seed=0)
+    losses = model.fit(feats, facs, steps=600, batch=4096, lr=1e-3)
+    assert losses[-10:].mean() < 0.05 * losses[:5].mean()
+    for s in (101, 102):
+        say("held-out seed", s)
+        t = tensor(s)
+        F = learned.features(t)
+        Ul, el = learned.hooi_trace(t, model.predict(F), iters, delta, stop_tol=1e-6)
+        Ur, er = learned.hooi_trace(t, np.random.default_rng(s).standard_normal((n, R)),
+                                    iters, delta, stop_tol=1e-6)
+        Uh0, _ = t.hoevd_init(R)
+        Uh, eh = learned.hooi_trace(t, Uh0, iters, delta, stop_tol=1e-6)
+        kl = learned.converged_at(Ul)
+        kr = learned.converged_at(Ur) or iters + 1
+        kh = learned.converged_at(Uh) or iters + 1
+        say("iterations learned/random/hoevd", kl, kr, kh,
+            "errors", el[-1], er[-1], eh[-1])
+        assert kl is not None and kl <= 10
+        assert kl < kr and kl < kh
+        # converged fit at least as good as where the fallbacks end
+        assert el[-1] <= min(er[-1], eh[-1]) + 1e-12
+        assert learned.sin_angle(Ul[-1], E) < 0.1
+
+
 def test_training_reduces_loss():
     rng = np.random.default_rng(3)
     X = rng.standard_normal((2000, 64)).astype(np.float32)

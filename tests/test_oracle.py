@@ -298,6 +298,84 @@ def test_s3ttmc_invariants():
     assert np.allclose(Yu @ Yu.T, (Y * W) @ Y.T, rtol=1e-12)
 
 
+# ---- HOEVD initialisation ----------------------------------------------------
+
+def dense_tensor(N, n, tt, v):
+    X = np.zeros((n,) * N)
+    for row, val in zip(np.asarray(tt) - 1, v):
+        for p in set(itertools.permutations(row)):
+            X[p] = val
+    return X
+
+
+def hoevd_gram_leaf_form(N, n, tt, v):
+    """A = sum over leaves S of mult(S) s_S s_S^T with the leaf's distinct
+    owners: the form the device kernel evaluates."""
+    T = oracle.Tree(N, n, tt, v)
+    lv = T.level(T.depth - 1)
+    rem, val = T.owners()
+    A = np.zeros((n, n))
+    for s, key in enumerate(lv["keys"]):
+        mult = math.factorial(N - 1) / np.prod(
+            [math.factorial(c) for c in np.unique(key, return_counts=True)[1]])
+        lo, hi = int(lv["level_ptr"][s]), int(lv["level_ptr"][s + 1])
+        own = {}
+        for o in range(lo, hi):
+            own[int(rem[o]) - 1] = val[o]   # a repeated owner is the same entry
+        idx = np.array(sorted(own))
+        sv = np.array([own[i] for i in idx])
+        A[np.ix_(idx, idx)] += mult * np.outer(sv, sv)
+    return A
+
+
+def test_hoevd_gram_is_the_unfolding_gram():
+    # spsymtensor.cpp:385-439 against X_(1) X_(1)^T of the dense tensor
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        N, n = int(rng.integers(2, 6)), int(rng.integers(2, 6))
+        tt = np.unique(np.sort(rng.integers(1, n + 1, size=(10, N)), axis=1), axis=0)
+        v = rng.standard_normal(len(tt))
+        A = oracle.hoevd_gram(N, n, tt, v)
+        X1 = dense_tensor(N, n, tt, v).reshape(n, -1)
+        assert np.allclose(A, X1 @ X1.T, rtol=1e-13, atol=1e-13)
+        assert np.array_equal(A, A.T)
+        # trace = sum of squares of all entries = ||X||^2
+        assert np.isclose(np.trace(A), oracle.norm(N, tt, v) ** 2, rtol=1e-13)
+
+
+def test_hoevd_gram_repeated_index_counts_once():
+    # (1,2,2,3): X_(1) row 2 has the columns (1,2,3) in 3! orderings, each once
+    tt = np.array([[1, 2, 2, 3]], np.uint32)
+    A = oracle.hoevd_gram(4, 3, tt, [2.0])
+    # row 1: columns = orderings of (2,2,3): 3 of them; row 2: of (1,2,3): 6;
+    # row 3: of (1,2,2): 3
+    assert np.allclose(np.diag(A), [3 * 4.0, 6 * 4.0, 3 * 4.0])
+    assert np.allclose(A - np.diag(np.diag(A)), 0.0)
+
+
+def test_hoevd_leaf_form_matches_coo_form():
+    rng = np.random.default_rng(12)
+    for _ in range(8):
+        N, n = int(rng.integers(3, 7)), int(rng.integers(3, 8))
+        tt = np.unique(np.sort(rng.integers(1, n + 1, size=(30, N)), axis=1), axis=0)
+        v = rng.random(len(tt))
+        assert np.allclose(hoevd_gram_leaf_form(N, n, tt, v),
+                           oracle.hoevd_gram(N, n, tt, v), rtol=1e-13, atol=1e-13)
+
+
+def test_hoevd_factor_is_top_eigenspace():
+    rng = np.random.default_rng(13)
+    N, n, R = 4, 9, 3
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(60, N)), axis=1), axis=0)
+    v = rng.random(len(tt))
+    A = oracle.hoevd_gram(N, n, tt, v)
+    U, w = oracle.hoevd_factor(A, R)
+    assert np.all(np.diff(w) >= 0)                      # ascending, as syevd's tail
+    assert np.allclose(A @ U, U * w, atol=1e-10 * w.max())
+    # the largest R of the whole spectrum
+    assert np.allclose(np.sort(np.linalg.eigvalsh(A))[-R:], w)
+
+
 def test_s3ttmc_threads_agree(contact5):
     o, d, inds, vals = contact5
     T = oracle.Tree(o, d, inds, vals)
