@@ -340,6 +340,60 @@ u32 orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   return good;
 }
 
+// One Cholesky-QR pass, U <- U L^{-T} with U^T U = L L^T (device Gram, host
+// Cholesky of the R x R Gram).  Two passes give Q of the thin QR with a
+// positive diagonal in R, orthonormal to working precision while cond(U) is
+// below ~1e7.  Returns false, leaving U untouched, when the Gram is not
+// safely positive definite (a pivot below 1e-14 of the largest diagonal).
+bool cholqr_pass(Ctx &c) {
+  const u32 R = c.R;
+  const u64 n = c.dim;
+  DevBuf &g = c.ws_rr;
+  g.reserve((u64)R * R * sizeof(double));
+  atb(c, c.d_U.as<double>(), R, c.d_U.as<double>(), R, n, g.as<double>(), "gram_u");
+  std::vector<double> G((size_t)R * R);
+  SP_HIP(hipMemcpyAsync(G.data(), g.p, G.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  double dmax = 0.0;
+  for (u32 i = 0; i < R; ++i) dmax = std::max(dmax, G[(size_t)i * R + i]);
+  if (!(dmax > 0.0) || !std::isfinite(dmax)) return false;
+  // lower Cholesky factor L (row-major), in place
+  std::vector<double> L((size_t)R * R, 0.0);
+  for (u32 j = 0; j < R; ++j) {
+    double d = G[(size_t)j * R + j];
+    for (u32 k = 0; k < j; ++k) d -= L[(size_t)j * R + k] * L[(size_t)j * R + k];
+    if (!(d > dmax * 1e-14)) return false;
+    const double ljj = std::sqrt(d);
+    L[(size_t)j * R + j] = ljj;
+    for (u32 i = j + 1; i < R; ++i) {
+      double s = G[(size_t)i * R + j];
+      for (u32 k = 0; k < j; ++k) s -= L[(size_t)i * R + k] * L[(size_t)j * R + k];
+      L[(size_t)i * R + j] = s / ljj;
+    }
+  }
+  // Linv = L^{-1} (lower), by forward substitution on the identity
+  std::vector<double> Li((size_t)R * R, 0.0);
+  for (u32 col = 0; col < R; ++col)
+    for (u32 i = col; i < R; ++i) {
+      double s = (i == col) ? 1.0 : 0.0;
+      for (u32 k = col; k < i; ++k) s -= L[(size_t)i * R + k] * Li[(size_t)k * R + col];
+      Li[(size_t)i * R + col] = s / L[(size_t)i * R + i];
+    }
+  // Q = U L^{-T}: Q[i][r] = sum_s U[i][s] Linv[r][s], i.e. M = Linv for y_times_mt
+  SP_HIP(hipMemcpyAsync(g.p, Li.data(), Li.size() * sizeof(double), hipMemcpyHostToDevice,
+                        c.stream));
+  DevBuf &tmp = c.ws_nr;
+  tmp.reserve(n * R * sizeof(double));
+  y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
+                                                   g.as<double>(), R, tmp.as<double>());
+  SP_CHECK_LAUNCH();
+  SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  return true;
+}
+
 }  // namespace
 
 // U's first `good` columns are orthonormal; replace the rest by an
@@ -470,7 +524,13 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   SP_HIP(hipStreamSynchronize(c.stream));
 }
 
+// Thin QR of U (matrix::qr, matrixops.cpp:58-75).  CholeskyQR2 (two Gram +
+// R x R Cholesky passes) gives the same Q as Householder up to column signs
+// (the sign/rotation freedom of quirk A14) at a fraction of rocSOLVER's
+// launch count; ill-conditioned U falls back to rocSOLVER geqrf + orgqr.
 void dense_qr(Ctx &c) {
+  ScopedTimer t(c, "qr");
+  if (cholqr_pass(c) && cholqr_pass(c)) return;
   const u64 n = c.dim;
   const u32 R = c.R;
   DevBuf &At = c.ws_nr;
@@ -481,7 +541,6 @@ void dense_qr(Ctx &c) {
                                                            At.as<double>());
   SP_CHECK_LAUNCH();
   rocblas_handle h = handle_for(c);
-  ScopedTimer t(c, "qr");
   if (rocsolver_dgeqrf(h, (rocblas_int)n, (rocblas_int)R, At.as<double>(), (rocblas_int)n,
                        tau.as<double>()) != rocblas_status_success)
     throw Error(SP_ERR_DEVICE, "rocsolver_dgeqrf failed");

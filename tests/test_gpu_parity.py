@@ -223,6 +223,25 @@ def test_dense_ops_vs_numpy():
     assert np.allclose(np.abs(Q.T @ Qref), np.eye(R), atol=1e-8)
 
 
+@pytest.mark.parametrize("cond", [1.0, 1e5, 1e10])
+def test_qr_well_and_ill_conditioned(cond):
+    # CholeskyQR2 for well-conditioned factors, Householder (rocSOLVER) past the
+    # Cholesky pivot guard; both must give Q = Householder Q up to column signs
+    rng = np.random.default_rng(31)
+    n, R = 3000, 8
+    t = sp.SpSymtensor(n, 3, np.array([[1, 2, 3]], np.uint32), np.ones(1))
+    A = np.linalg.qr(rng.standard_normal((n, R)))[0]
+    B = np.linalg.qr(rng.standard_normal((R, R)))[0]
+    U = (A * np.logspace(0, -np.log10(cond), R)) @ B
+    t.set_factor(U)
+    say("qr cond", cond)
+    t.qr()
+    Q = t.factor()
+    assert np.allclose(Q.T @ Q, np.eye(R), atol=1e-12)
+    Qref = np.linalg.qr(U)[0]
+    assert np.allclose(np.abs(Q.T @ Qref), np.eye(R), atol=1e-5 if cond > 1e6 else 1e-10)
+
+
 @pytest.mark.parametrize("eig", ["syevd", "syevj"])
 def test_leftsvd_subspace(eig, monkeypatch):
     monkeypatch.setenv("SP_EIG", eig)
