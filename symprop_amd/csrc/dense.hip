@@ -29,30 +29,54 @@ namespace {
 // as numerically absent (singular value ratio ~3e-6).
 constexpr double kGramRankTol = 1e-11;
 
-constexpr int RB = 16;  // rank columns per register block
+typedef double d4 __attribute__((ext_vector_type(4)));
 
-// partial[b][r][p] = sum over rows of block b of A[i][r] * B[i][p]
+// v_mfma_f64_16x16x4f64 operand layout: lane l holds A[m = l%16][k = l/16]
+// and B[k = l/16][n = l%16]; the accumulator's register q holds
+// C[m = 4q + l/16][n = l%16].
+__device__ inline d4 mfma64(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// partial[b][r][p] = sum over rows i of block b of A[i][r] * B[i][p]
+// (A: n x R, B: n x P, both row-major).  FP64 MFMA: a wavefront owns 16
+// columns of B and one 16-wide block of r, and walks its rows four at a time
+// (K = rows); two accumulators alternate to keep the matrix pipe busy.
+// grid (row blocks, ceil(P/64)); 4 wavefronts per block.
 __global__ void __launch_bounds__(256)
 atb_partial(const double *__restrict__ A, u32 R, const double *__restrict__ B,
             u64 P, u64 n, u64 rows_per, double *__restrict__ partial) {
-  const u64 p = (u64)blockIdx.y * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kr = lane >> 4, kc = lane & 15;
+  const u64 col = (u64)blockIdx.y * 64 + wave * 16 + kc;
   const u64 i0 = (u64)blockIdx.x * rows_per;
   const u64 i1 = min(n, i0 + rows_per);
-  for (u32 r0 = 0; r0 < R; r0 += RB) {
-    double acc[RB];
+  const bool cin = col < P;
+  for (u32 r0 = 0; r0 < R; r0 += 16) {
+    const bool rin = r0 + kc < R;
+    d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    u64 i = i0;
+    for (; i + 8 <= i1; i += 8) {
+      const u64 ia = i + kr, ib = i + 4 + kr;
+      const double a0 = rin ? A[ia * R + r0 + kc] : 0.0;
+      const double b0 = cin ? B[ia * P + col] : 0.0;
+      const double a1 = rin ? A[ib * R + r0 + kc] : 0.0;
+      const double b1 = cin ? B[ib * P + col] : 0.0;
+      acc0 = mfma64(a0, b0, acc0);
+      acc1 = mfma64(a1, b1, acc1);
+    }
+    for (; i < i1; i += 4) {
+      const u64 ia = i + kr;
+      const bool in = ia < i1;
+      const double a0 = (rin && in) ? A[ia * R + r0 + kc] : 0.0;
+      const double b0 = (cin && in) ? B[ia * P + col] : 0.0;
+      acc0 = mfma64(a0, b0, acc0);
+    }
+    // C[m = 4q + kr][n = kc]: m is r - r0, n the column
 #pragma unroll
-    for (int q = 0; q < RB; ++q) acc[q] = 0.0;
-    if (p < P) {
-      for (u64 i = i0; i < i1; ++i) {
-        const double y = B[i * P + p];
-        const double *a = A + i * R + r0;
-#pragma unroll
-        for (int q = 0; q < RB; ++q)
-          if (r0 + q < R) acc[q] += a[q] * y;
-      }
-#pragma unroll
-      for (int q = 0; q < RB; ++q)
-        if (r0 + q < R) partial[((u64)blockIdx.x * R + r0 + q) * P + p] = acc[q];
+    for (int q = 0; q < 4; ++q) {
+      const u32 r = r0 + 4 * q + kr;
+      if (r < R && cin) partial[((u64)blockIdx.x * R + r) * P + col] = acc0[q] + acc1[q];
     }
   }
 }
@@ -66,31 +90,51 @@ __global__ void sum_partials(const double *__restrict__ partial, u64 nb, u64 len
   out[t] = s;
 }
 
-// out[i][r] = sum_p Y[i][p] * M[r][p]; one wave per row.
+// out[i][r] = sum_p Y[i][p] * M[r][p]  (Y: n x P, M: R x P, out: n x R).
+// FP64 MFMA with K = P: a block stages a 64-row x 32-column slab of Y and the
+// matching 16 x 32 slab of M in LDS (coalesced row reads), each wavefront
+// multiplies its 16 rows.  grid (ceil(n/64), ceil(R/16)).
+constexpr int YK = 32;
 __global__ void __launch_bounds__(256)
 y_times_mt(const double *__restrict__ Y, u64 n, u64 P, const double *__restrict__ M,
            u32 R, double *__restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const u64 i = (u64)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
-  for (u32 r0 = 0; r0 < R; r0 += RB) {
-    double acc[RB];
-#pragma unroll
-    for (int q = 0; q < RB; ++q) acc[q] = 0.0;
-    for (u64 p = lane; p < P; p += 64) {
-      const double y = Y[i * P + p];
-#pragma unroll
-      for (int q = 0; q < RB; ++q)
-        if (r0 + q < R) acc[q] += y * M[(u64)(r0 + q) * P + p];
+  __shared__ double sy[64][YK + 1];
+  __shared__ double sm[16][YK + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kr = lane >> 4, kc = lane & 15;
+  const u64 row0 = (u64)blockIdx.x * 64;
+  const u32 r0 = blockIdx.y * 16;
+  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  for (u64 k0 = 0; k0 < P; k0 += YK) {
+    for (int t = threadIdx.x; t < 64 * YK; t += 256) {
+      const int rr = t / YK, kk = t % YK;
+      const u64 i = row0 + rr, k = k0 + kk;
+      sy[rr][kk] = (i < n && k < P) ? Y[i * P + k] : 0.0;
     }
-#pragma unroll
-    for (int q = 0; q < RB; ++q) {
-      double v = acc[q];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0 && r0 + q < R) out[i * R + r0 + q] = v;
+    for (int t = threadIdx.x; t < 16 * YK; t += 256) {
+      const int rr = t / YK, kk = t % YK;
+      const u64 k = k0 + kk;
+      sm[rr][kk] = (r0 + rr < R && k < P) ? M[(u64)(r0 + rr) * P + k] : 0.0;
     }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < YK / 4; ks += 2) {
+      // A[m][k] = Y[row0 + 16 wave + m][k0 + 4 ks + k], B[k][nn] = M[r0 + nn][k0 + 4 ks + k]
+      acc0 = mfma64(sy[wave * 16 + kc][4 * ks + kr], sm[kc][4 * ks + kr], acc0);
+      acc1 = mfma64(sy[wave * 16 + kc][4 * ks + 4 + kr], sm[kc][4 * ks + 4 + kr], acc1);
+    }
+    __syncthreads();
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const u64 i = row0 + wave * 16 + 4 * q + kr;
+    const u32 r = r0 + kc;
+    if (i < n && r < R) out[i * R + r] = acc0[q] + acc1[q];
+  }
+}
+
+inline dim3 ymt_grid(u64 n, u32 R) {
+  return dim3((unsigned)((n + 63) / 64), (R + 15) / 16);
 }
 
 __global__ void scale_cols(const double *__restrict__ core, const double *__restrict__ W,
@@ -101,46 +145,73 @@ __global__ void scale_cols(const double *__restrict__ core, const double *__rest
 
 // Gram tile kernel: partial[kb][tile] for 64x64 tiles (tp <= tq) of Y^T Y.
 constexpr int GT = 64;
-constexpr int GK = 16;
+constexpr int GK = 32;
+constexpr int GL = GK * GT / 256;  // slab elements per thread
 __global__ void __launch_bounds__(256)
 gram_partial(const double *__restrict__ Y, u64 n, u64 P, u64 rows_per,
              const int2 *__restrict__ tiles, double *__restrict__ partial) {
+  // FP64 MFMA: wavefront w computes the 32 x 32 quadrant (w/2, w%2) of the
+  // 64 x 64 tile as 2 x 2 blocks of v_mfma_f64_16x16x4f64 with K = rows.
+  // The next GK-row slab is loaded into registers while the current one in
+  // LDS is multiplied.
   __shared__ double sa[GK][GT + 1];
   __shared__ double sb[GK][GT + 1];
   const int2 tl = tiles[blockIdx.x];
   const u64 p0 = (u64)tl.x * GT, q0 = (u64)tl.y * GT;
+  const bool diag = tl.x == tl.y;
   const u64 i0 = (u64)blockIdx.y * rows_per;
   const u64 i1 = min(n, i0 + rows_per);
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4] = {};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kr = lane >> 4, kc = lane & 15;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  // this thread's slab elements: row kk = t / GT, column c = t % GT with
+  // t = threadIdx.x + 256 j (c is the same for every j)
+  const int c = threadIdx.x % GT, kk0 = threadIdx.x / GT;
+  const bool ca = p0 + c < P, cb = q0 + c < P;
+  double ra[GL], rb[GL];
+  auto fetch = [&](u64 k) {
+#pragma unroll
+    for (int j = 0; j < GL; ++j) {
+      const u64 i = k + kk0 + 4 * j;
+      ra[j] = (i < i1 && ca) ? Y[i * P + p0 + c] : 0.0;
+      rb[j] = (!diag && i < i1 && cb) ? Y[i * P + q0 + c] : 0.0;
+    }
+  };
+  d4 acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = d4{0.0, 0.0, 0.0, 0.0};
+  if (i0 < i1) fetch(i0);
   for (u64 k = i0; k < i1; k += GK) {
-    for (int t = threadIdx.x; t < GK * GT; t += 256) {
-      const int kk = t / GT, c = t % GT;
-      const u64 i = k + kk;
-      sa[kk][c] = (i < i1 && p0 + c < P) ? Y[i * P + p0 + c] : 0.0;
-      sb[kk][c] = (i < i1 && q0 + c < P) ? Y[i * P + q0 + c] : 0.0;
+#pragma unroll
+    for (int j = 0; j < GL; ++j) {
+      sa[kk0 + 4 * j][c] = ra[j];
+      sb[kk0 + 4 * j][c] = rb[j];
     }
     __syncthreads();
+    if (k + GK < i1) fetch(k + GK);
+    double(*bs)[GT + 1] = diag ? sa : sb;
 #pragma unroll
-    for (int kk = 0; kk < GK; ++kk) {
-      double a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[u] = sa[kk][ty + 16 * u];
-        b[u] = sb[kk][tx + 16 * u];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    for (int ks = 0; ks < GK / 4; ++ks) {
+      const int kk = 4 * ks + kr;
+      const double a0 = sa[kk][wr + kc], a1 = sa[kk][wr + 16 + kc];
+      const double b0 = bs[kk][wc + kc], b1 = bs[kk][wc + 16 + kc];
+      acc[0][0] = mfma64(a0, b0, acc[0][0]);
+      acc[0][1] = mfma64(a0, b1, acc[0][1]);
+      acc[1][0] = mfma64(a1, b0, acc[1][0]);
+      acc[1][1] = mfma64(a1, b1, acc[1][1]);
     }
     __syncthreads();
   }
   double *out = partial + ((u64)blockIdx.y * gridDim.x + blockIdx.x) * GT * GT;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) out[(ty + 16 * u) * GT + tx + 16 * v] = acc[u][v];
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        out[(wr + 16 * u + 4 * q + kr) * GT + wc + 16 * v + kc] = acc[u][v][q];
 }
 
 __global__ void gram_finish(const double *__restrict__ partial, u64 nk, u64 ntiles,
@@ -153,6 +224,7 @@ __global__ void gram_finish(const double *__restrict__ partial, u64 nk, u64 ntil
   const int2 tl = tiles[tile];
   const u64 p = (u64)tl.x * GT + u, q = (u64)tl.y * GT + v;
   if (p >= P || q >= P) return;
+  if (tl.x == tl.y && u > v) return;  // each symmetric pair written once
   double s = 0.0;
   for (u64 k = 0; k < nk; ++k) s += partial[(k * ntiles + tile) * GT * GT + u * GT + v];
   s *= sqrt(W[p]) * sqrt(W[q]);
@@ -277,9 +349,12 @@ rocblas_handle handle_for(Ctx &c) {
 // out = A^T B over n rows (A: n x R, B: n x P) -> R x P, deterministic.
 void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *out,
          const char *name) {
-  const u64 ctiles = (P + 255) / 256;
-  u64 nb = std::max<u64>(1, std::min<u64>(512, (n + 127) / 128));
-  const u64 rows_per = (n + nb - 1) / nb;
+  const u64 ctiles = (P + 63) / 64;
+  // about 2048 blocks in all but at most 256 row blocks (the partial sums
+  // are added serially), at least 32 rows each, rows a multiple of 8
+  u64 nb = std::max<u64>(1, std::min<u64>(std::min<u64>(2048 / ctiles + 1, 256),
+                                          (n + 31) / 32));
+  const u64 rows_per = ((n + nb - 1) / nb + 7) / 8 * 8;
   nb = (n + rows_per - 1) / rows_per;
   c.d_scratch.reserve(nb * R * P * sizeof(double));
   {
@@ -331,7 +406,7 @@ u32 orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   SP_HIP(hipMemcpyAsync(g.p, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice,
                         c.stream));
   tmp.reserve(n * R * sizeof(double));
-  y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
+  y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
                                                    g.as<double>(), R, tmp.as<double>());
   SP_CHECK_LAUNCH();
   SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
@@ -385,7 +460,7 @@ bool cholqr_pass(Ctx &c) {
                         c.stream));
   DevBuf &tmp = c.ws_nr;
   tmp.reserve(n * R * sizeof(double));
-  y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
+  y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
                                                    g.as<double>(), R, tmp.as<double>());
   SP_CHECK_LAUNCH();
   SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
@@ -429,7 +504,7 @@ void dense_symgemm(Ctx &c) {
   scale_cols<<<grid_for((u64)R * c.P, 256), 256, 0, c.stream>>>(
       c.d_core.as<double>(), c.d_W.as<double>(), R, c.P, M.as<double>());
   SP_CHECK_LAUNCH();
-  y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, c.P,
+  y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, c.P,
                                                    M.as<double>(), R, out.as<double>());
   SP_CHECK_LAUNCH();
   SP_HIP(hipMemcpyAsync(c.d_U.p, out.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
@@ -444,8 +519,9 @@ void dense_gram(Ctx &c) {
   for (u64 a = 0; a < nt; ++a)
     for (u64 b = a; b < nt; ++b) tiles.push_back(make_int2((int)a, (int)b));
   const u64 ntiles = tiles.size();
-  u64 nk = std::max<u64>(1, std::min<u64>((n + 255) / 256, 1024 / ntiles + 1));
-  const u64 rows_per = (n + nk - 1) / nk;
+  // about 2048 blocks in all, at least 128 rows each, rows a multiple of GK
+  u64 nk = std::max<u64>(1, std::min<u64>((n + 127) / 128, 2048 / ntiles + 1));
+  const u64 rows_per = ((n + nk - 1) / nk + GK - 1) / GK * GK;
   nk = (n + rows_per - 1) / rows_per;
   DevBuf &dt = c.ws_tiles;
   dt.reserve(ntiles * sizeof(int2));
@@ -516,7 +592,7 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   SP_CHECK_LAUNCH();
   {
     ScopedTimer t(c, "y_times_v");
-    y_times_mt<<<grid_for(n, 4), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, P,
+    y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, P,
                                                      M.as<double>(), R, c.d_U.as<double>());
     SP_CHECK_LAUNCH();
   }
