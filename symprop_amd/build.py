@@ -21,9 +21,11 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 COMMON = ["-O3", "-fPIC", "-std=c++20", "-Wall", "-Wno-unused-result",
           "-I" + os.path.join(HERE, "..", "include"), "-I" + CSRC, "-I" + HOST]
-# Packed FP32 VALU forms are disabled for all device code.
+# Packed FP32 VALU forms are disabled for all device code.  MFMA operands and
+# accumulators are kept in VGPRs: with the default heuristic the accumulators
+# of the S3TTMc kernels were copied between VGPRs and AGPRs around every MFMA.
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
-          "-packed-fp32-ops"]
+          "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _sources():

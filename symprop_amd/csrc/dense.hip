@@ -232,16 +232,26 @@ __global__ void gram_finish(const double *__restrict__ partial, u64 nk, u64 ntil
   G[q * P + p] = s;
 }
 
-// M[r][p] = sqrt(W[p]) * V[:, P-1-r][p] / sqrt(lambda_{P-1-r})
-__global__ void svd_map(const double *__restrict__ V, const double *__restrict__ lam,
+// M[r][p] = sqrt(W[p]) * V[p][r] / sqrt(theta_r)  (V: P x R row-major, the
+// top-R eigenvectors of G, theta descending)
+__global__ void svd_map(const double *__restrict__ V, const double *__restrict__ theta,
                         const double *__restrict__ W, u64 P, u32 R, double floor_,
                         double *__restrict__ M) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (u64)R * P) return;
   const u64 r = t / P, p = t % P;
-  const u64 j = P - 1 - r;
-  const double l = lam[j] > floor_ ? lam[j] : floor_;
-  M[t] = sqrt(W[p]) * V[j * P + p] / sqrt(l);
+  const double l = theta[r] > floor_ ? theta[r] : floor_;
+  M[t] = sqrt(W[p]) * V[p * R + r] / sqrt(l);
+}
+
+// V[p][r] = column P-1-r of the column-major eigenvector matrix Z (the
+// eigenvector of the r-th largest eigenvalue from an ascending solver)
+__global__ void top_columns(const double *__restrict__ Z, u64 P, u32 R,
+                            double *__restrict__ V) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * R) return;
+  const u64 p = t / R, r = t % R;
+  V[t] = Z[(P - 1 - r) * P + p];
 }
 
 __global__ void transpose_nr(const double *__restrict__ in, u64 rows, u64 cols,
@@ -542,17 +552,64 @@ void dense_gram(Ctx &c) {
   SP_HIP(hipStreamSynchronize(c.stream));
 }
 
+void dense_eig_full(Ctx &c, double *D, double *E, double *resid, rocblas_int *info);
+
 void dense_leftsvd_y(Ctx &c, double *sigma) {
   dense_gram(c);
   const u64 P = c.P, n = c.dim;
   const u32 R = c.R;
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds packed width");
-  // workspace: D (P), E (P), residual (1), then two int words (info, sweeps)
-  c.ws_eig.reserve((2 * P + 2) * sizeof(double) + 4 * sizeof(rocblas_int));
+  // workspace: D (P), E (P), residual (1), theta (R), then int words
+  c.ws_eig.reserve((2 * P + 2 + R) * sizeof(double) + 4 * sizeof(rocblas_int));
   double *D = c.ws_eig.as<double>();
   double *E = D + P;
   double *resid = E + P;
-  rocblas_int *info = reinterpret_cast<rocblas_int *>(resid + 2);
+  double *theta_d = resid + 2;
+  rocblas_int *info = reinterpret_cast<rocblas_int *>(theta_d + R);
+  // top-R eigenvectors of G as V (P x R, row-major), eigenvalues descending
+  DevBuf &V = c.ws_pr;
+  V.reserve(P * R * sizeof(double));
+  std::vector<double> theta(R);
+  {
+    dense_eig_full(c, D, E, resid, info);
+    std::vector<double> lam(P);
+    SP_HIP(hipMemcpyAsync(lam.data(), D, P * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+    for (u32 r = 0; r < R; ++r) theta[r] = lam[P - 1 - r];
+    top_columns<<<grid_for(P * R, 256), 256, 0, c.stream>>>(c.d_G.as<double>(), P, R,
+                                                           V.as<double>());
+    SP_CHECK_LAUNCH();
+  }
+  if (sigma)
+    for (u32 r = 0; r < R; ++r) sigma[r] = std::sqrt(std::max(theta[r], 0.0));
+  // The Gram's eigenvalues carry absolute errors of about eps * lambda_max,
+  // so directions with lambda below kGramRankTol * lambda_max (singular value
+  // below ~3e-6 of the largest), and all of them for a zero Y, have no
+  // reliable left vector; they get an orthonormal completion
+  u32 good = 0;
+  while (good < R && theta[good] > 0.0 && theta[good] > theta[0] * kGramRankTol) ++good;
+  const double lmax = std::max(std::fabs(theta[0]), 1e-300);
+  SP_HIP(hipMemcpyAsync(theta_d, theta.data(), R * sizeof(double), hipMemcpyHostToDevice,
+                        c.stream));
+  DevBuf &M = c.ws_rp;
+  M.reserve((u64)R * P * sizeof(double));
+  svd_map<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
+      V.as<double>(), theta_d, c.d_W.as<double>(), P, R, lmax * 1e-30, M.as<double>());
+  SP_CHECK_LAUNCH();
+  {
+    ScopedTimer t(c, "y_times_v");
+    y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, P,
+                                                     M.as<double>(), R, c.d_U.as<double>());
+    SP_CHECK_LAUNCH();
+  }
+  complete_basis(c, good);
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+// All eigenpairs of G by rocSOLVER (syevd, or syevj when selected):
+// eigenvectors overwrite G column-major in ascending order, values in D.
+void dense_eig_full(Ctx &c, double *D, double *E, double *resid, rocblas_int *info) {
+  const u64 P = c.P;
   rocblas_handle h = handle_for(c);
   if (c.eig_method == 1) {
     ScopedTimer t(c, "syevj");
@@ -568,36 +625,10 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
                                          (rocblas_int)P, D, E, info);
     if (st != rocblas_status_success) throw Error(SP_ERR_DEVICE, "rocsolver_dsyevd failed");
   }
-  std::vector<double> lam(P);
   rocblas_int hinfo = 0;
-  SP_HIP(hipMemcpyAsync(lam.data(), D, P * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   SP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
   if (hinfo != 0) throw Error(SP_ERR_NUMERIC, "eigensolver did not converge");
-  const double lmax = std::max(std::fabs(lam[P - 1]), 1e-300);
-  if (sigma)
-    for (u32 r = 0; r < R; ++r) sigma[r] = std::sqrt(std::max(lam[P - 1 - r], 0.0));
-  // The Gram's eigenvalues carry absolute errors of about eps * lambda_max,
-  // so directions with lambda below kGramRankTol * lambda_max (singular value
-  // below ~3e-6 of the largest), and all of them for a zero Y, have no
-  // reliable left vector; they get an orthonormal completion
-  u32 good = 0;
-  while (good < R && lam[P - 1 - good] > 0.0 &&
-         lam[P - 1 - good] > lam[P - 1] * kGramRankTol)
-    ++good;
-  DevBuf &M = c.ws_rp;
-  M.reserve((u64)R * P * sizeof(double));
-  svd_map<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
-      c.d_G.as<double>(), D, c.d_W.as<double>(), P, R, lmax * 1e-30, M.as<double>());
-  SP_CHECK_LAUNCH();
-  {
-    ScopedTimer t(c, "y_times_v");
-    y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_Y.as<double>(), n, P,
-                                                     M.as<double>(), R, c.d_U.as<double>());
-    SP_CHECK_LAUNCH();
-  }
-  complete_basis(c, good);
-  SP_HIP(hipStreamSynchronize(c.stream));
 }
 
 // Thin QR of U (matrix::qr, matrixops.cpp:58-75).  CholeskyQR2 (two Gram +

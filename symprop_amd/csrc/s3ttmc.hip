@@ -305,7 +305,7 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
                    hipMemcpyHostToDevice));
 
   // specialised MFMA kernel for this (order, rank), with its fold table
-  c.fast_kernel = fast_kernel_for(m, R, &c.fast_H, &c.fast_kernel_bf16);
+  c.fast_kernel = fast_kernel_for(m, R, &c.fast_H, &c.fast_kernel_f32, &c.fast_kernel_bf16);
   if (c.fast_kernel) {
     std::vector<u32> fptr, fterm;
     make_fold_table(R, m, c.fast_H, fptr, fterm);
@@ -377,6 +377,18 @@ void s3ttmc_run(Ctx &c, int precision) {
     launch_fast(c.fast_kernel, nchunks, c.stream, c.d_chunks.as<Chunk>(),
                 c.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(),
                 c.d_U.as<double>(), nullptr, nullptr, c.d_fold_ptr.as<u32>(),
+                c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
+                c.d_partial.as<double>());
+    SP_CHECK_LAUNCH();
+  } else if (precision == SP_PREC_FP32 && c.fast_kernel_f32) {
+    c.d_U32.reserve(nU * sizeof(float));
+    convert_factor<float><<<grid_for(nU, 256), 256, 0, c.stream>>>(
+        c.d_U.as<double>(), nU, c.d_U32.as<float>());
+    SP_CHECK_LAUNCH();
+    ScopedTimer t(c, "s3ttmc_mfma_f32");
+    launch_fast(c.fast_kernel_f32, nchunks, c.stream, c.d_chunks.as<Chunk>(),
+                c.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
+                c.w_coef32.as<float>(), c.d_U32.as<float>(), c.d_fold_ptr.as<u32>(),
                 c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
                 c.d_partial.as<double>());
     SP_CHECK_LAUNCH();

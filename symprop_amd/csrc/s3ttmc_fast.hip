@@ -160,6 +160,38 @@ struct FastArgs {
 };
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Element type of the fast kernel: FP64 operands on v_mfma_f64_16x16x4f64,
+// or FP32 operands on v_mfma_f32_16x16x4f32.  Both take lane l's A[l%16][l/16]
+// and B[l/16][l%16]; accumulator register q holds C[4q + l/16][l%16] for f64
+// and C[4(l/16) + q][l%16] for f32.
+template <class T>
+struct FastElem;
+template <>
+struct FastElem<double> {
+  using acc_t = d4;
+  static __device__ __forceinline__ acc_t mma(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int crow(int q, int kr) { return 4 * q + kr; }
+  static __device__ __forceinline__ const double *coef(const struct FastArgs &a);
+  static __device__ __forceinline__ const double *U(const struct FastArgs &a);
+};
+template <>
+struct FastElem<float> {
+  using acc_t = f4;
+  static __device__ __forceinline__ acc_t mma(float x, float y, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int crow(int q, int kr) { return 4 * kr + q; }
+  static __device__ __forceinline__ const float *coef(const struct FastArgs &a);
+  static __device__ __forceinline__ const float *U(const struct FastArgs &a);
+};
+__device__ __forceinline__ const double *FastElem<double>::coef(const FastArgs &a) { return a.coef; }
+__device__ __forceinline__ const double *FastElem<double>::U(const FastArgs &a) { return a.U; }
+__device__ __forceinline__ const float *FastElem<float>::coef(const FastArgs &a) { return a.coef32; }
+__device__ __forceinline__ const float *FastElem<float>::U(const FastArgs &a) { return a.U32; }
 
 struct BatchV {
   u64 o0;
@@ -206,24 +238,28 @@ __device__ __forceinline__ void load_rows(const T *U, const u32 (&tp)[M], T (&ro
   }
 }
 
-template <int M, int R, int H>
+// TE = double: the FP64 form; TE = float: the FP32 mixed form (FP32 U rows,
+// coefficients, products and accumulation; the fold sums in FP64).
+template <int M, int R, int H, class TE>
 __global__ void __launch_bounds__(256)
-s3ttmc_mfma64(FastArgs a) {
+s3ttmc_mfma(FastArgs a) {
+  using EL = FastElem<TE>;
+  using acc_t = typename EL::acc_t;
   constexpr int T = M - H;
   constexpr int PH = psz(R, H), PT = psz(R, T), PM = psz(R, M);
   constexpr int TH = (PH + 15) / 16, TT = (PT + 15) / 16;
   constexpr int NTILE = TH * TT;
   constexpr int TPW = (NTILE + 3) / 4;
   constexpr int BO = 64;
-  constexpr int LDA = TH * 16 + 1;  // odd stride in doubles: spreads the 4 row groups over banks
+  constexpr int LDA = TH * 16 + 1;  // odd stride: spreads the 4 row groups over banks
   constexpr int LDB = TT * 16 + 1;
   constexpr int SA = BO * LDA, SB = BO * LDB;
   constexpr int SF = PH * PT;
   constexpr int SMEM = (SA + SB) > SF ? (SA + SB) : SF;
-  __shared__ double sm[SMEM];
-  double *sA = sm;
-  double *sB = sm + SA;
-  double *sF = sm;
+  __shared__ TE sm[SMEM];
+  TE *sA = sm;
+  TE *sB = sm + SA;
+  TE *sF = sm;
 
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -249,23 +285,25 @@ s3ttmc_mfma64(FastArgs a) {
   // to its last batch (valid memory, results unused).
   const u64 blast = ch.b_end - 1;
   u32 tpn[M];
-  double cur_c, nxt_cv;
+  TE cur_c, nxt_cv;
   bool nxt_in;
-  double rows[M * R];
+  TE rows[M * R];
+  const TE *coef = EL::coef(a);
+  const TE *Urows = EL::U(a);
   {
     u32 tp0[M];
-    double cv0;
+    TE cv0;
     bool in0;
-    load_owner<M>(a, a.coef, read_batch(a.batches, ch.b_begin), lane, tp0, cv0, in0);
-    cur_c = in0 ? cv0 : 0.0;
-    load_rows<M, R>(a.U, tp0, rows);
-    load_owner<M>(a, a.coef, read_batch(a.batches, min(ch.b_begin + 1, blast)), lane, tpn,
+    load_owner<M>(a, coef, read_batch(a.batches, ch.b_begin), lane, tp0, cv0, in0);
+    cur_c = in0 ? cv0 : TE(0);
+    load_rows<M, R>(Urows, tp0, rows);
+    load_owner<M>(a, coef, read_batch(a.batches, min(ch.b_begin + 1, blast)), lane, tpn,
                   nxt_cv, nxt_in);
   }
 
-  d4 acc[TPW];
+  acc_t acc[TPW];
 #pragma unroll
-  for (int i = 0; i < TPW; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < TPW; ++i) acc[i] = acc_t{0, 0, 0, 0};
 
   for (u64 b = ch.b_begin; b < ch.b_end; ++b) {
     const BatchV bt = read_batch(a.batches, b);
@@ -273,29 +311,28 @@ s3ttmc_mfma64(FastArgs a) {
     const int ksteps = (nb + 3) >> 2;
     // ---- producer: lane = owner, wave = quarter of the coefficients
     if (lane < ksteps * 4) {
-      double *arow = sA + lane * LDA;
-      double *brow = sB + lane * LDB;
+      TE *arow = sA + lane * LDA;
+      TE *brow = sB + lane * LDB;
       // padding lanes (lane >= nb) write zero rows: both scales are 0
-      const double one = lane < nb ? 1.0 : 0.0;
+      const TE one = lane < nb ? TE(1) : TE(0);
       produce_quarters<R, H, T>(
-          wave, rows, cur_c, one, [&](int J, double v) { arow[J] = v; },
-          [&](int J, double v) { brow[J] = v; }, TH * 16, TT * 16);
+          wave, rows, cur_c, one, [&](int J, TE v) { arow[J] = v; },
+          [&](int J, TE v) { brow[J] = v; }, TH * 16, TT * 16);
     }
     __syncthreads();
     // ---- prefetch: U rows of batch b+1, tuple of batch b+2
-    cur_c = nxt_in ? nxt_cv : 0.0;
-    load_rows<M, R>(a.U, tpn, rows);
-    load_owner<M>(a, a.coef, read_batch(a.batches, min(b + 2, blast)), lane, tpn, nxt_cv,
+    cur_c = nxt_in ? nxt_cv : TE(0);
+    load_rows<M, R>(Urows, tpn, rows);
+    load_owner<M>(a, coef, read_batch(a.batches, min(b + 2, blast)), lane, tpn, nxt_cv,
                   nxt_in);
     // ---- consumer: MFMA over the batch
     for (int ks = 0; ks < ksteps; ++ks) {
-      const double *ar = sA + ks * 4 * LDA;
-      const double *br = sB + ks * 4 * LDB;
+      const TE *ar = sA + ks * 4 * LDA;
+      const TE *br = sB + ks * 4 * LDB;
 #pragma unroll
       for (int i = 0; i < TPW; ++i) {
         if (NTILE % 4 == 0 || wave + 4 * i < NTILE)
-          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[aoff[i]], br[boff[i]], acc[i], 0,
-                                                        0, 0);
+          acc[i] = EL::mma(ar[aoff[i]], br[boff[i]], acc[i]);
       }
     }
     __syncthreads();
@@ -309,11 +346,11 @@ s3ttmc_mfma64(FastArgs a) {
         const int t = tt * 16 + kc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int h = th * 16 + 4 * q + kr;
+          const int h = th * 16 + EL::crow(q, kr);
           if (h < PH && t < PT) sF[h * PT + t] = acc[i][q];
         }
       }
-      acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+      acc[i] = acc_t{0, 0, 0, 0};
     }
     __syncthreads();
     double *out = (ch.partial >= 0) ? a.partial + (u64)ch.partial * PM : a.Y + (u64)bt.row * PM;
@@ -321,7 +358,7 @@ s3ttmc_mfma64(FastArgs a) {
       double s = 0.0;
       for (u32 q = a.fold_ptr[J]; q < a.fold_ptr[J + 1]; ++q) {
         const u32 e = a.fold_term[q];
-        s += sF[(e >> 16) * PT + (e & 0xFFFFu)];
+        s += (double)sF[(e >> 16) * PT + (e & 0xFFFFu)];
       }
       out[J] = s * a.fact[J];
     }
@@ -330,9 +367,8 @@ s3ttmc_mfma64(FastArgs a) {
 }
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
 
-// Mixed-precision form of s3ttmc_mfma64: FP32 U rows, coefficients and
+// Mixed-precision form of s3ttmc_mfma: FP32 U rows, coefficients and
 // polynomial products; A = c*head and B = tail rounded to BF16 (round to
 // nearest even) and multiplied on v_mfma_f32_16x16x32_bf16 with FP32
 // accumulation; the fold sums in FP64.  Operands are stored transposed
@@ -447,11 +483,14 @@ using FastKernel = void (*)(FastArgs);
 
 struct FastEntry {
   int M, R, H;
-  FastKernel k64, kbf16;
+  FastKernel k64, kf32, kbf16;
 };
 
-#define SP_FAST(M, R, H) \
-  FastEntry { M, R, H, s3ttmc_mfma64<M, R, H>, s3ttmc_mfma_bf16<M, R, H> }
+#define SP_FAST(M, R, H)                                                        \
+  FastEntry {                                                                   \
+    M, R, H, s3ttmc_mfma<M, R, H, double>, s3ttmc_mfma<M, R, H, float>,         \
+        s3ttmc_mfma_bf16<M, R, H>                                               \
+  }
 
 const FastEntry kFast[] = {
     // order 3
@@ -513,11 +552,13 @@ void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32
   }
 }
 
-// Returns the instantiated fast kernel for (M, R) or null; sets H.
-void *fast_kernel_for(int M, u32 R, int *H, void **kbf16) {
+// Returns the instantiated FP64 fast kernel for (M, R) or null; sets H and
+// the FP32 and BF16 forms of the same shape.
+void *fast_kernel_for(int M, u32 R, int *H, void **kf32, void **kbf16) {
   for (const auto &e : kFast)
     if (e.M == M && (u32)e.R == R) {
       *H = e.H;
+      if (kf32) *kf32 = (void *)e.kf32;
       if (kbf16) *kbf16 = (void *)e.kbf16;
       return (void *)e.k64;
     }

@@ -159,6 +159,7 @@ struct Ctx {
   DevBuf d_partial;
   // persistent workspaces of the dense Y-side ops (grown, never shrunk)
   DevBuf ws_rp;    // R x P
+  DevBuf ws_pr;    // P x R (top eigenvectors of the Gram)
   DevBuf ws_nr;    // n x R
   DevBuf ws_rr;    // R x R
   DevBuf ws_eig;   // 2P + R doubles + info words
@@ -180,6 +181,7 @@ struct Ctx {
 
   // compile-time specialised MFMA kernel for (order-1, R), if instantiated
   void *fast_kernel = nullptr;
+  void *fast_kernel_f32 = nullptr;
   void *fast_kernel_bf16 = nullptr;
   int fast_H = 0;
   DevBuf d_fold_ptr, d_fold_term;
@@ -207,7 +209,7 @@ void radix_sort(K *keys, K *keys_tmp, u32 *vals, u32 *vals_tmp, u64 n,
 void s3ttmc_prepare(Ctx &c, u32 R);
 void s3ttmc_run(Ctx &c, int precision);
 void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
-void *fast_kernel_for(int M, u32 R, int *H, void **kbf16);
+void *fast_kernel_for(int M, u32 R, int *H, void **kf32, void **kbf16);
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
                  const Batch *batches, const u32 *tup, const double *coef, const double *U,
                  const float *coef32, const float *U32, const u32 *fold_ptr,

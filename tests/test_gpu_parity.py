@@ -178,6 +178,27 @@ def test_s3ttmc_fp32(contact5):
 
 
 @pytest.mark.parametrize("N,n,nnz,R,seed", [(4, 2000, 40000, 10, 1), (5, 300, 6000, 8, 2),
+                                            (6, 400, 5000, 8, 3), (7, 60, 1500, 6, 4),
+                                            (3, 500, 8000, 4, 5)])
+def test_s3ttmc_fp32_mfma(N, n, nnz, R, seed):
+    # FP32 operands and accumulation on v_mfma_f32_16x16x4f32: FP32-level
+    # error, and the MFMA kernel (not the generic one) ran
+    inds, vals = sp.generate(sp.GEN_ZIPF, N, n, nnz, 1.1, 1, seed=seed)
+    U = np.linalg.qr(np.random.default_rng(seed).standard_normal((n, R)))[0]
+    say("s3ttmc fp32 mfma", N, n, len(vals), R)
+    t = sp.SpSymtensor(n, N, inds, vals)
+    Y32 = t.s3ttmc(U, precision=sp.PREC_FP32)
+    Y64 = t.s3ttmc(U)
+    e = werr(Y32, Y64, R, N)
+    assert 1e-12 < e < 2e-5, e
+    t.timing(True)
+    t.s3ttmc_dev(sp.PREC_FP32)
+    assert t.timing_get("s3ttmc_mfma_f32")[1] == 1
+    # deterministic
+    assert np.array_equal(t.s3ttmc(U, precision=sp.PREC_FP32), Y32)
+
+
+@pytest.mark.parametrize("N,n,nnz,R,seed", [(4, 2000, 40000, 10, 1), (5, 300, 6000, 8, 2),
                                             (6, 400, 5000, 8, 3), (7, 60, 1500, 6, 4)])
 def test_s3ttmc_bf16_mfma(N, n, nnz, R, seed):
     # BF16 operands, FP32 accumulation: error well below 1e-2 and well above
