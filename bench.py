@@ -162,6 +162,48 @@ def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0)
                       % (threads, len(errs))}
 
 
+def dense_y_side(t, N, n, R, delta, iters=3):
+    """One HOOI iteration's device work, op by op (HIP events on the library
+    stream), and the MFMA utilisation of the Y-side contractions: the Gram
+    G = diag(sqrt W) Y^T Y diag(sqrt W) (64 x 64 upper tiles, FP64 MFMA),
+    U^T Y and Y M^T."""
+    P = binom(R + N - 2, N - 1)
+    t.timing(True)
+    t.timing_reset()
+    for _ in range(iters):
+        t.regularize(delta)
+        t.s3ttmc_dev()
+        t.leftsvd_y()
+        t.gemmtn()
+    t.sync()
+    ops = {}
+    for name in ("s3ttmc_mfma64", "s3ttmc_generic", "reduce_split_rows", "gram", "syevd",
+                 "y_times_v", "gemmtn", "gram_u", "qr"):
+        ms, cnt = t.timing_get(name)
+        if cnt:
+            ops[name] = ms / iters
+    t.timing(False)
+    nt = (P + 63) // 64
+    tiles = nt * (nt + 1) // 2
+    g_ms = ops.get("gram", 0.0)
+    out = {"ms_per_iteration": ops, "P": P}
+    if g_ms > 0:
+        executed = tiles * 64 * 64 * 2.0 * n   # MFMA flops issued (upper tiles, padded)
+        algorithmic = 2.0 * n * P * P          # the full P x P product
+        out["gram"] = {"kernel_ms": g_ms,
+                       "mfma_tflops": executed / (g_ms * 1e-3) / 1e12,
+                       "mfma_utilisation": executed / (g_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                       "alg_tflops": algorithmic / (g_ms * 1e-3) / 1e12,
+                       "peak_tflops": FP64_PEAK_TFLOPS}
+    y_bytes = 8.0 * n * P
+    for name in ("gemmtn", "y_times_v"):
+        if ops.get(name):
+            out[name] = {"kernel_ms": ops[name],
+                         "hbm_gbs": y_bytes / (ops[name] * 1e-3) / 1e9,
+                         "hbm_frac": y_bytes / (ops[name] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def planted_basis(n, K):
     E = np.zeros((n, K))
     b = n // K
@@ -371,6 +413,12 @@ def main():
                      "kernel_ms": k_avg, "alg_flops": F,
                      "split_reduce_ms": r_ms / max(1, r_cnt)},
     }
+
+    if traffic:
+        gbs = traffic / (k_avg * 1e-3) / 1e9
+        out["roofline"]["hbm_achieved_gbs"] = gbs
+        out["roofline"]["hbm_frac"] = gbs / HBM_PEAK_GBS
+    out["dense_y_side"] = dense_y_side(t, N, n, R, w["delta"])
 
     if not args.no_hooi:
         runs = {}
