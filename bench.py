@@ -132,7 +132,37 @@ def cpu_baseline(inds, vals, order, dim, R, budget_s=25.0):
             "insertion_s": t_build, "s3ttmc_s": t_s3}
 
 
-def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0):
+def cpu_hooi_estimate(order, dim, nnz, R, s3ttmc_rate, threads):
+    """Where the literal CPU HOOI cannot run (its unpacked n x R^(N-1) Y and
+    the SVD of it), the strongest CPU form with the same arithmetic: the
+    reference's own S3TTMc (opt5) at the rate measured on a bounded sample of
+    this tensor, plus a packed-Gram HOOI iteration (P x P Gram, eigh,
+    U = Y V, core) timed with numpy's multi-threaded LAPACK on an n x P
+    matrix of the same shape."""
+    P = binom(R + order - 2, order - 1)
+    rng = np.random.default_rng(0)
+    Y = rng.standard_normal((dim, P))
+    w = rng.random(P) + 0.5
+    t0 = time.perf_counter()
+    Z = Y * np.sqrt(w)
+    G = Z.T @ Z
+    lam, V = np.linalg.eigh(G)
+    U = Z @ (V[:, -R:] / np.sqrt(np.maximum(lam[-R:], 1e-300)))
+    core = U.T @ Y
+    t_dense = time.perf_counter() - t0
+    t_s3 = nnz / s3ttmc_rate
+    return {"kind": "estimate", "cores": threads,
+            "literal_infeasible": "the reference HOOI unpacks Y to %d x %d FP64 (%.1f GB) and "
+                                  "takes its SVD each iteration" % (dim, R ** (order - 1),
+                                                                    dim * R ** (order - 1) * 8e-9),
+            "s_per_iter": t_s3 + t_dense, "s3ttmc_s": t_s3, "dense_s": t_dense,
+            "sample": "reference opt5 rate %.3e nnz/s from the bounded cpu_baseline sample "
+                      "x %d nnz, plus a numpy packed-Gram iteration at n=%d, P=%d"
+                      % (s3ttmc_rate, nnz, dim, P), "core_shape": list(core.shape)}
+
+
+def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0,
+                      s3ttmc_rate=None):
     """HOOI on the host cores: the oracle's restatement of TuckerHOOI
     (S3TTMc in the reference's opt5 schedule with OpenMP, numpy LAPACK for
     the SVDs), same tensor, rank and random start.  Bounded: if one
@@ -144,6 +174,10 @@ def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0)
     except Exception as e:  # pragma: no cover
         return {"error": "oracle unavailable: %s" % e}
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1
+    if dim * R ** (order - 1) * 8 > 2e9:
+        if not s3ttmc_rate:
+            return {"error": "literal CPU HOOI infeasible and no CPU S3TTMc rate measured"}
+        return cpu_hooi_estimate(order, dim, len(vals), R, s3ttmc_rate, threads)
     t0 = time.perf_counter()
     T = oracle.Tree(order, dim, inds, vals)
     t_build = time.perf_counter() - t0
@@ -420,12 +454,24 @@ def main():
         out["roofline"]["hbm_frac"] = gbs / HBM_PEAK_GBS
     out["dense_y_side"] = dense_y_side(t, N, n, R, w["delta"])
 
+    hooi_fixed = None
     if not args.no_hooi:
+        # the configuration's fixed iteration count, FP64 (compared with the
+        # CPU HOOI baseline over the same iterations)
+        r = t.tucker(alg=0, rank=R, max_iter=w["max_iter"], rel_tol=0.0, delta=w["delta"],
+                     seed=0, precision=sp.PREC_FP64)
+        hooi_fixed = {"iters": r["iters"], "seconds": r["seconds"],
+                      "s_per_iter": r["seconds"] / max(1, r["iters"]),
+                      "final_error": float(r["errors"][-1])}
+        out["hooi_fixed_iterations"] = hooi_fixed
+        # time to fit: every form runs TuckerHOOI's own stopping rule
+        # |error - previous| < 1e-6 (spsymtensor.cpp:235-238), up to 500
+        # iterations; the mixed forms switch to FP64 sweeps on the way
         runs = {}
         for name, prec in (("fp64", sp.PREC_FP64), ("fp32_mixed", sp.PREC_FP32),
                            ("bf16_mixed", sp.PREC_BF16)):
             try:
-                r = t.tucker(alg=0, rank=R, max_iter=w["max_iter"], rel_tol=1e-6,
+                r = t.tucker(alg=0, rank=R, max_iter=500, rel_tol=1e-6,
                              delta=w["delta"], seed=0, precision=prec)
             except sp.SpError as e:
                 runs[name] = {"error": str(e)}
@@ -452,8 +498,23 @@ def main():
         out["cpu_baseline"] = cpu_baseline(inds, vals, N, n, R)
         if not args.no_hooi:
             log("cpu HOOI baseline (oracle loop on host cores)")
-            out["cpu_hooi_baseline"] = cpu_hooi_baseline(inds, vals, N, n, R, w["max_iter"],
-                                                         w["delta"])
+            cb = cpu_hooi_baseline(inds, vals, N, n, R, w["max_iter"], w["delta"],
+                                   s3ttmc_rate=out["cpu_baseline"].get("value"))
+            out["cpu_hooi_baseline"] = cb
+            if "s_per_iter" in cb and hooi_fixed:
+                cb["gpu_fp64_speedup_per_iteration"] = cb["s_per_iter"] / hooi_fixed["s_per_iter"]
+                f64 = out.get("hooi_time_to_fit", {}).get("fp64", {})
+                if "iters" in f64:
+                    # the CPU loop runs the same iteration sequence (same start,
+                    # same stopping rule), so its time to fit is its
+                    # per-iteration time times the FP64 iteration count
+                    cb["time_to_fit_estimate_s"] = cb["s_per_iter"] * f64["iters"]
+                    cb["time_to_fit_estimate_note"] = (
+                        "CPU s/iteration x the %d iterations the FP64 run needed" % f64["iters"])
+                    for k, v in out["hooi_time_to_fit"].items():
+                        if "seconds" in v:
+                            v["speedup_vs_cpu_time_to_fit"] = (cb["time_to_fit_estimate_s"] /
+                                                               v["seconds"])
     print(json.dumps(out), flush=True)
 
 
