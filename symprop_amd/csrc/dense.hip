@@ -8,7 +8,8 @@
 //   leftsvd  top-R left singular vectors of the unpacked Y: with
 //            Z = Y diag(sqrt W), Z Z^T = Y_unpacked Y_unpacked^T, so
 //            U = Z V_R Lambda_R^{-1/2} from the top-R eigenpairs of G
-//   qr       thin Q of U (rocSOLVER geqrf/orgqr)  (src/dense/matrixops.cpp:58-75)
+//   qr       thin Q of U by Cholesky-QR2, rocSOLVER geqrf/orgqr when U is
+//            ill-conditioned           (src/dense/matrixops.cpp:58-75)
 //   regularize  row clip to delta + left singular vectors of U
 //            (src/sparse/spsymtensor.cpp:168-199)
 // Reductions over the n rows go through fixed-order partial sums, so results

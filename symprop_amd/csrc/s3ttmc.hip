@@ -210,8 +210,8 @@ __global__ void convert_factor(const double *__restrict__ U, u64 n, T *__restric
   if (i < n) out[i] = (T)U[i];
 }
 
-// Batches of up to 64 owners of one row for the MFMA kernel.
-void push_batches(std::vector<Batch> &out, u32 row, u64 ob, u64 oe) {
+// Batches of up to `bo` owners of one row for the MFMA kernel.
+void push_batches(std::vector<Batch> &out, u32 row, u64 ob, u64 oe, u32 bo) {
   auto mk = [&](u64 o, u32 nb, bool last) {
     out.push_back(Batch{(u32)o, (u32)(o >> 32), row, nb | (last ? 0x80000000u : 0u)});
   };
@@ -219,16 +219,16 @@ void push_batches(std::vector<Batch> &out, u32 row, u64 ob, u64 oe) {
     mk(ob, 0, true);
     return;
   }
-  for (u64 o = ob; o < oe; o += 64) {
-    const u64 e = std::min(oe, o + 64);
+  for (u64 o = ob; o < oe; o += bo) {
+    const u64 e = std::min(oe, o + bo);
     mk(o, (u32)(e - o), e == oe);
   }
 }
 
-void build_chunks(Ctx &c, u64 target) {
-  c.h_chunks.clear();
-  c.h_split.clear();
-  c.h_batches.clear();
+void build_chunks(Ctx &c, Plan &pl, u64 target, u32 bo) {
+  pl.h_chunks.clear();
+  pl.h_split.clear();
+  pl.h_batches.clear();
   const auto &sp = c.h_segptr;
   const u32 n = c.dim;
   u64 slot = 0;
@@ -240,35 +240,49 @@ void build_chunks(Ctx &c, u64 target) {
       SplitRow s{r, (u32)slot, 0};
       for (u64 o = sp[r]; o < sp[r + 1]; o += target) {
         const u64 e = std::min(sp[r + 1], o + target);
-        const u64 b0 = c.h_batches.size();
-        push_batches(c.h_batches, r, o, e);
-        c.h_chunks.push_back(Chunk{o, e, r, r + 1, (i64)slot++, b0, c.h_batches.size()});
+        const u64 b0 = pl.h_batches.size();
+        push_batches(pl.h_batches, r, o, e, bo);
+        pl.h_chunks.push_back(Chunk{o, e, r, r + 1, (i64)slot++, b0, pl.h_batches.size()});
       }
       s.slot_end = (u32)slot;
-      c.h_split.push_back(s);
+      pl.h_split.push_back(s);
       ++r;
       continue;
     }
     const u32 start = r;
     u64 acc = 0;
-    const u64 b0 = c.h_batches.size();
+    const u64 b0 = pl.h_batches.size();
     while (r < n && r - start < MAX_ROWS) {
       const u64 k = sp[r + 1] - sp[r];
       if (k > target || (acc + k > target && r > start)) break;
       acc += k;
-      push_batches(c.h_batches, r, sp[r], sp[r + 1]);
+      push_batches(pl.h_batches, r, sp[r], sp[r + 1], bo);
       ++r;
     }
-    c.h_chunks.push_back(Chunk{sp[start], sp[r], start, r, -1, b0, c.h_batches.size()});
+    pl.h_chunks.push_back(Chunk{sp[start], sp[r], start, r, -1, b0, pl.h_batches.size()});
   }
-  c.n_partial = slot;
+  pl.n_partial = slot;
+}
+
+
+void upload_plan(Plan &pl) {
+  pl.d_chunks.alloc(std::max<size_t>(1, pl.h_chunks.size()) * sizeof(Chunk));
+  SP_HIP(hipMemcpy(pl.d_chunks.p, pl.h_chunks.data(), pl.h_chunks.size() * sizeof(Chunk),
+                   hipMemcpyHostToDevice));
+  pl.d_batches.alloc(std::max<size_t>(1, pl.h_batches.size()) * sizeof(Batch));
+  SP_HIP(hipMemcpy(pl.d_batches.p, pl.h_batches.data(), pl.h_batches.size() * sizeof(Batch),
+                   hipMemcpyHostToDevice));
+  pl.d_split.alloc(std::max<size_t>(1, pl.h_split.size()) * sizeof(SplitRow));
+  if (!pl.h_split.empty())
+    SP_HIP(hipMemcpy(pl.d_split.p, pl.h_split.data(), pl.h_split.size() * sizeof(SplitRow),
+                     hipMemcpyHostToDevice));
 }
 
 }  // namespace
 
 void s3ttmc_prepare(Ctx &c, u32 R) {
   if (!c.built) throw Error(SP_ERR_STATE, "tree not built");
-  if (c.chunk_R == R && !c.h_chunks.empty()) return;
+  if (c.chunk_R == R && !c.plan[0].h_chunks.empty()) return;
   const int m = (int)c.order - 1;
   if (m > 15) throw Error(SP_ERR_UNSUPPORTED, "order above 16");
   if (R == 0 || R > 255) throw Error(SP_ERR_UNSUPPORTED, "rank must be in [1,255]");
@@ -305,7 +319,9 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
                    hipMemcpyHostToDevice));
 
   // specialised MFMA kernel for this (order, rank), with its fold table
-  c.fast_kernel = fast_kernel_for(m, R, &c.fast_H, &c.fast_kernel_f32, &c.fast_kernel_bf16);
+  c.fast_bo = 64;
+  c.fast_kernel = fast_kernel_for(m, R, &c.fast_H, &c.fast_bo, &c.fast_kernel_f32,
+                                  &c.fast_kernel_bf16);
   if (c.fast_kernel) {
     std::vector<u32> fptr, fterm;
     make_fold_table(R, m, c.fast_H, fptr, fterm);
@@ -322,18 +338,16 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
   const u64 target =
       c.fast_kernel ? std::max<u64>(256, std::min<u64>(4096, c.n_owners / 2048 + 1))
                     : std::max<u64>(16, std::min<u64>(2048, c.n_owners / 8192 + 1));
-  build_chunks(c, target);
-  c.d_chunks.alloc(c.h_chunks.size() * sizeof(Chunk));
-  SP_HIP(hipMemcpy(c.d_chunks.p, c.h_chunks.data(), c.h_chunks.size() * sizeof(Chunk),
-                   hipMemcpyHostToDevice));
-  c.d_batches.alloc(std::max<size_t>(1, c.h_batches.size()) * sizeof(Batch));
-  SP_HIP(hipMemcpy(c.d_batches.p, c.h_batches.data(), c.h_batches.size() * sizeof(Batch),
-                   hipMemcpyHostToDevice));
-  c.d_split.alloc(std::max<size_t>(1, c.h_split.size()) * sizeof(SplitRow));
-  if (!c.h_split.empty())
-    SP_HIP(hipMemcpy(c.d_split.p, c.h_split.data(), c.h_split.size() * sizeof(SplitRow),
-                     hipMemcpyHostToDevice));
-  c.d_partial.reserve(std::max<u64>(1, c.n_partial) * c.P * sizeof(double));
+  build_chunks(c, c.plan[0], target, 64);
+  upload_plan(c.plan[0]);
+  u64 np = c.plan[0].n_partial;
+  c.plan[1] = Plan{};
+  if (c.fast_kernel && c.fast_bo < 64) {
+    build_chunks(c, c.plan[1], target, (u32)c.fast_bo);
+    upload_plan(c.plan[1]);
+    np = std::max(np, c.plan[1].n_partial);
+  }
+  c.d_partial.reserve(std::max<u64>(1, np) * c.P * sizeof(double));
   c.chunk_R = R;
 }
 
@@ -345,7 +359,6 @@ void s3ttmc_run(Ctx &c, int precision) {
   c.d_Y.reserve((u64)c.dim * c.P * sizeof(double));
 
   GenArgs a{};
-  a.chunks = c.d_chunks.as<Chunk>();
   a.tup = c.w_tuple.as<u32>();
   a.segptr = c.w_segptr.as<u64>();
   a.tab_ptr = c.d_tab_ptr.as<u32>();
@@ -370,12 +383,15 @@ void s3ttmc_run(Ctx &c, int precision) {
   a.P = c.P;
   a.Y = c.d_Y.as<double>();
   a.partial = c.d_partial.as<double>();
-  const unsigned nchunks = (unsigned)c.h_chunks.size();
+  const Plan &pl = (precision == SP_PREC_FP64 && c.fast_kernel && c.fast_bo < 64) ? c.plan[1]
+                                                                               : c.plan[0];
+  a.chunks = pl.d_chunks.as<Chunk>();
+  const unsigned nchunks = (unsigned)pl.h_chunks.size();
 
   if (precision == SP_PREC_FP64 && c.fast_kernel) {
     ScopedTimer t(c, "s3ttmc_mfma64");
-    launch_fast(c.fast_kernel, nchunks, c.stream, c.d_chunks.as<Chunk>(),
-                c.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(),
+    launch_fast(c.fast_kernel, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
+                pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(),
                 c.d_U.as<double>(), nullptr, nullptr, c.d_fold_ptr.as<u32>(),
                 c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
                 c.d_partial.as<double>());
@@ -386,8 +402,8 @@ void s3ttmc_run(Ctx &c, int precision) {
         c.d_U.as<double>(), nU, c.d_U32.as<float>());
     SP_CHECK_LAUNCH();
     ScopedTimer t(c, "s3ttmc_mfma_f32");
-    launch_fast(c.fast_kernel_f32, nchunks, c.stream, c.d_chunks.as<Chunk>(),
-                c.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
+    launch_fast(c.fast_kernel_f32, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
+                pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
                 c.w_coef32.as<float>(), c.d_U32.as<float>(), c.d_fold_ptr.as<u32>(),
                 c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
                 c.d_partial.as<double>());
@@ -398,8 +414,8 @@ void s3ttmc_run(Ctx &c, int precision) {
         c.d_U.as<double>(), nU, c.d_U32.as<float>());
     SP_CHECK_LAUNCH();
     ScopedTimer t(c, "s3ttmc_mfma_bf16");
-    launch_fast(c.fast_kernel_bf16, nchunks, c.stream, c.d_chunks.as<Chunk>(),
-                c.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
+    launch_fast(c.fast_kernel_bf16, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
+                pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
                 c.w_coef32.as<float>(), c.d_U32.as<float>(), c.d_fold_ptr.as<u32>(),
                 c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
                 c.d_partial.as<double>());
@@ -423,11 +439,11 @@ void s3ttmc_run(Ctx &c, int precision) {
         a, c.d_U32.as<float>(), c.w_coef32.as<float>());
     SP_CHECK_LAUNCH();
   }
-  if (!c.h_split.empty()) {
+  if (!pl.h_split.empty()) {
     ScopedTimer t(c, "reduce_split_rows");
-    reduce_split_rows<<<dim3((unsigned)c.h_split.size(), (unsigned)((c.P + 63) / 64)), 64, 0,
+    reduce_split_rows<<<dim3((unsigned)pl.h_split.size(), (unsigned)((c.P + 63) / 64)), 64, 0,
                         c.stream>>>(
-        c.d_split.as<SplitRow>(), c.d_partial.as<double>(), c.P, c.d_Y.as<double>());
+        pl.d_split.as<SplitRow>(), c.d_partial.as<double>(), c.P, c.d_Y.as<double>());
     SP_CHECK_LAUNCH();
   }
 }

@@ -10,7 +10,7 @@
 //     Ypre (P_H x P_T) = A^T B,   A = [c_o head_o],  B = [tail_o],
 // run on FP64 MFMA (v_mfma_f64_16x16x4f64, 16x16 tiles, K = 4 owners).
 //
-// Work split of a 256-thread workgroup over a batch of 64 owners of one row:
+// Work split of a 256-thread workgroup over a batch of 64 (or 48) owners of one row:
 //   producer: lane = owner, wave w computes the w-th quarter of the head and
 //             tail coefficients in registers (fully unrolled, constant
 //             indices) and stores them to LDS;
@@ -238,10 +238,26 @@ __device__ __forceinline__ void load_rows(const T *U, const u32 (&tp)[M], T (&ro
   }
 }
 
+// Owners per batch of the FP64 kernel: 64, or 48 where the FP64
+// operand tiles of 64 owners would exceed 80 KB of LDS, so that two
+// workgroups fit a CU (160 KB) and one's producer phase overlaps the other's
+// MFMA phase.
+constexpr int fast_bo(int M, int R, int H) {
+  return 64 * ((psz(R, H) + 15) / 16 * 16 + 1 + (psz(R, M - H) + 15) / 16 * 16 + 1) * 8 >
+                 80 * 1024
+             ? 48
+             : 64;
+}
+
 // TE = double: the FP64 form; TE = float: the FP32 mixed form (FP32 U rows,
 // coefficients, products and accumulation; the fold sums in FP64).
 template <int M, int R, int H, class TE>
-__global__ void __launch_bounds__(256)
+constexpr int kernel_bo() {
+  return sizeof(TE) == 8 ? fast_bo(M, R, H) : 64;  // the FP32 form runs 64-owner batches
+}
+
+template <int M, int R, int H, class TE>
+__global__ void __launch_bounds__(256, (kernel_bo<M, R, H, TE>() < 64 ? 2 : 1))
 s3ttmc_mfma(FastArgs a) {
   using EL = FastElem<TE>;
   using acc_t = typename EL::acc_t;
@@ -250,7 +266,7 @@ s3ttmc_mfma(FastArgs a) {
   constexpr int TH = (PH + 15) / 16, TT = (PT + 15) / 16;
   constexpr int NTILE = TH * TT;
   constexpr int TPW = (NTILE + 3) / 4;
-  constexpr int BO = 64;
+  constexpr int BO = kernel_bo<M, R, H, TE>();
   constexpr int LDA = TH * 16 + 1;  // odd stride: spreads the 4 row groups over banks
   constexpr int LDB = TT * 16 + 1;
   constexpr int SA = BO * LDA, SB = BO * LDB;
@@ -483,13 +499,14 @@ using FastKernel = void (*)(FastArgs);
 
 struct FastEntry {
   int M, R, H;
+  int bo;  // owners per batch
   FastKernel k64, kf32, kbf16;
 };
 
 #define SP_FAST(M, R, H)                                                        \
   FastEntry {                                                                   \
-    M, R, H, s3ttmc_mfma<M, R, H, double>, s3ttmc_mfma<M, R, H, float>,         \
-        s3ttmc_mfma_bf16<M, R, H>                                               \
+    M, R, H, fast_bo(M, R, H), s3ttmc_mfma<M, R, H, double>,                     \
+        s3ttmc_mfma<M, R, H, float>, s3ttmc_mfma_bf16<M, R, H>                  \
   }
 
 const FastEntry kFast[] = {
@@ -554,10 +571,11 @@ void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32
 
 // Returns the instantiated FP64 fast kernel for (M, R) or null; sets H and
 // the FP32 and BF16 forms of the same shape.
-void *fast_kernel_for(int M, u32 R, int *H, void **kf32, void **kbf16) {
+void *fast_kernel_for(int M, u32 R, int *H, int *bo, void **kf32, void **kbf16) {
   for (const auto &e : kFast)
     if (e.M == M && (u32)e.R == R) {
       *H = e.H;
+      if (bo) *bo = e.bo;
       if (kf32) *kf32 = (void *)e.kf32;
       if (kbf16) *kbf16 = (void *)e.kbf16;
       return (void *)e.k64;

@@ -97,7 +97,7 @@ struct Chunk {
   u64 b_begin, b_end;  // batch range (MFMA kernel)
 };
 
-// Up to 64 consecutive owners of one Y row; `last` marks the batch after
+// Up to 64 (the fast kernel's batch size) consecutive owners of one Y row; `last` marks the batch after
 // which the row is complete (an empty row has one batch with nb = 0).
 // All fields are dwords so that the kernel reads a batch with scalar loads.
 struct Batch {
@@ -109,6 +109,15 @@ struct Batch {
 struct SplitRow {
   u32 row;
   u32 slot_begin, slot_end;
+};
+
+// Work decomposition of the rem-sorted owners for one batch size.
+struct Plan {
+  std::vector<Chunk> h_chunks;
+  std::vector<SplitRow> h_split;
+  std::vector<Batch> h_batches;
+  DevBuf d_chunks, d_split, d_batches;
+  u64 n_partial = 0;
 };
 
 struct Timer {
@@ -166,12 +175,10 @@ struct Ctx {
   DevBuf ws_tiles; // Gram tile list
   int eig_method = 0;  // 0: rocSOLVER syevd (divide and conquer), 1: syevj (Jacobi)
 
-  // chunking of the work list for the current R
-  std::vector<Chunk> h_chunks;
-  std::vector<SplitRow> h_split;
-  std::vector<Batch> h_batches;
-  DevBuf d_chunks, d_split, d_batches;
-  u64 n_partial = 0;
+  // chunking of the work list for the current R: plan[0] with 64-owner
+  // batches (generic, FP32 and BF16 kernels), plan[1] with the FP64 fast
+  // kernel's batch size when that is smaller
+  Plan plan[2];
   u32 chunk_R = 0;
 
   // tables per level for the current R
@@ -184,6 +191,7 @@ struct Ctx {
   void *fast_kernel_f32 = nullptr;
   void *fast_kernel_bf16 = nullptr;
   int fast_H = 0;
+  int fast_bo = 64;  // owners per batch of the fast kernels
   DevBuf d_fold_ptr, d_fold_term;
 
   // profiling
@@ -209,7 +217,7 @@ void radix_sort(K *keys, K *keys_tmp, u32 *vals, u32 *vals_tmp, u64 n,
 void s3ttmc_prepare(Ctx &c, u32 R);
 void s3ttmc_run(Ctx &c, int precision);
 void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
-void *fast_kernel_for(int M, u32 R, int *H, void **kf32, void **kbf16);
+void *fast_kernel_for(int M, u32 R, int *H, int *bo, void **kf32, void **kbf16);
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
                  const Batch *batches, const u32 *tup, const double *coef, const double *U,
                  const float *coef32, const float *U32, const u32 *fold_ptr,
