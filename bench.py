@@ -62,6 +62,11 @@ def workload(name):
         return dict(kind=sp.GEN_PLANTED, order=6, dim=50000, nnz=5_000_000, param=0.8,
                     communities=8, rank=8, max_iter=20, delta=0.5,
                     desc="order-6 n=50000 5M unique planted-8-community tuples, rank 8")
+    if name == "zipf7":
+        return dict(kind=sp.GEN_ZIPF, order=7, dim=100000, nnz=10_000_000, param=1.2,
+                    communities=1, rank=6, max_iter=20, delta=0.5, alg=1,
+                    desc="order-7 n=100000 10M unique Zipf(1.2) tuples U(0,1) values, rank 6, "
+                         "HOQRI")
     if name == "uniform5":
         return dict(kind=sp.GEN_UNIFORM, order=5, dim=327, nnz=8000, param=0.0,
                     communities=1, rank=8, max_iter=20, delta=0.1,
@@ -159,6 +164,77 @@ def cpu_hooi_estimate(order, dim, nnz, R, s3ttmc_rate, threads):
             "sample": "reference opt5 rate %.3e nnz/s from the bounded cpu_baseline sample "
                       "x %d nnz, plus a numpy packed-Gram iteration at n=%d, P=%d"
                       % (s3ttmc_rate, nnz, dim, P), "core_shape": list(core.shape)}
+
+
+def opt5_level_bytes(t, N, R):
+    """FP64 bytes of the level arrays the reference's kronecker_product_opt5
+    materialises (sum over levels 1..N-2 of L_l * P_{l+1} * 8), from the
+    level sizes of the tree built on the GPU."""
+    return sum(t.level_size(l) * binom(R + l, l + 1) * 8 for l in range(1, N - 1))
+
+
+def host_memory_bytes():
+    try:
+        return os.sysconf("SC_PAGE_SIZE") * os.sysconf("SC_PHYS_PAGES")
+    except (ValueError, OSError, AttributeError):  # pragma: no cover
+        return 0
+
+
+def cpu_hoqri_estimate(order, dim, nnz, R, s3ttmc_rate, threads, level_bytes):
+    """CPU HOQRI per iteration where the reference's S3TTMc cannot run on the
+    host (its opt5 level arrays exceed host memory): the reference's opt5 at
+    the rate measured on a bounded prefix of this tensor, plus the dense part
+    of a TuckerHOQRI iteration (core = U^T Y, U = Y diag(W) core^T, thin QR;
+    spsymtensor.cpp:254-299) timed with numpy's multi-threaded BLAS/LAPACK on
+    matrices of the same shape."""
+    P = binom(R + order - 2, order - 1)
+    rng = np.random.default_rng(0)
+    Y = rng.standard_normal((dim, P))
+    w = rng.random(P) + 0.5
+    U = np.linalg.qr(rng.standard_normal((dim, R)))[0]
+    t0 = time.perf_counter()
+    core = U.T @ Y
+    Un = Y @ (w[:, None] * core.T)
+    U, _ = np.linalg.qr(Un)
+    t_dense = time.perf_counter() - t0
+    t_s3 = nnz / s3ttmc_rate
+    return {"kind": "estimate", "cores": threads,
+            "literal_infeasible": "one reference S3TTMc of the whole tensor takes %.0f s on "
+                                  "the host cores and materialises %.0f GB of FP64 level arrays "
+                                  "(host memory %.0f GB)"
+                                  % (nnz / s3ttmc_rate, level_bytes * 1e-9,
+                                     host_memory_bytes() * 1e-9),
+            "s_per_iter": t_s3 + t_dense, "s3ttmc_s": t_s3, "dense_s": t_dense,
+            "sample": "reference opt5 rate %.3e nnz/s from the bounded cpu_baseline sample "
+                      "x %d nnz, plus a numpy HOQRI dense step at n=%d, P=%d"
+                      % (s3ttmc_rate, nnz, dim, P)}
+
+
+def cpu_hoqri_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0):
+    """HOQRI on the host cores: the oracle's restatement of TuckerHOQRI
+    (opt5 S3TTMc with OpenMP, numpy BLAS/LAPACK for core, Y W core^T and the
+    QR), same tensor, rank and random start, bounded like cpu_hooi_baseline."""
+    try:
+        import oracle
+        from oracle import tucker
+    except Exception as e:  # pragma: no cover
+        return {"error": "oracle unavailable: %s" % e}
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1
+    t0 = time.perf_counter()
+    T = oracle.Tree(order, dim, inds, vals)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tucker.hoqri(T, R, 1, rel_tol=0.0, delta=delta, seed=0)
+    t_one = time.perf_counter() - t0
+    iters = max(1, min(max_iter, int(budget_s / max(t_one, 1e-3))))
+    t0 = time.perf_counter()
+    _, errs, _ = tucker.hoqri(T, R, iters, rel_tol=1e-6, delta=delta, seed=0)
+    t_run = time.perf_counter() - t0
+    return {"kind": "port", "cores": threads, "construction_s": t_build,
+            "iters": len(errs), "seconds": t_run, "s_per_iter": t_run / len(errs),
+            "final_error": float(errs[-1]),
+            "sample": "oracle TuckerHOQRI restatement (opt5 S3TTMc, OpenMP; numpy QR), "
+                      "%d iterations" % len(errs)}
 
 
 def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0,
@@ -454,24 +530,27 @@ def main():
         out["roofline"]["hbm_frac"] = gbs / HBM_PEAK_GBS
     out["dense_y_side"] = dense_y_side(t, N, n, R, w["delta"])
 
+    alg = w.get("alg", 0)
+    aname = ("hooi", "hoqri")[alg]
+    level_bytes = opt5_level_bytes(t, N, R)
     hooi_fixed = None
     if not args.no_hooi:
         # the configuration's fixed iteration count, FP64 (compared with the
-        # CPU HOOI baseline over the same iterations)
-        r = t.tucker(alg=0, rank=R, max_iter=w["max_iter"], rel_tol=0.0, delta=w["delta"],
+        # CPU baseline over the same iterations)
+        r = t.tucker(alg=alg, rank=R, max_iter=w["max_iter"], rel_tol=0.0, delta=w["delta"],
                      seed=0, precision=sp.PREC_FP64)
         hooi_fixed = {"iters": r["iters"], "seconds": r["seconds"],
                       "s_per_iter": r["seconds"] / max(1, r["iters"]),
                       "final_error": float(r["errors"][-1])}
-        out["hooi_fixed_iterations"] = hooi_fixed
-        # time to fit: every form runs TuckerHOOI's own stopping rule
-        # |error - previous| < 1e-6 (spsymtensor.cpp:235-238), up to 500
-        # iterations; the mixed forms switch to FP64 sweeps on the way
+        out["%s_fixed_iterations" % aname] = hooi_fixed
+        # time to fit: every form runs the reference loop's own stopping rule
+        # |error - previous| < 1e-6 (spsymtensor.cpp:235-238, 283-286), up to
+        # 500 iterations; the mixed forms switch to FP64 sweeps on the way
         runs = {}
         for name, prec in (("fp64", sp.PREC_FP64), ("fp32_mixed", sp.PREC_FP32),
                            ("bf16_mixed", sp.PREC_BF16)):
             try:
-                r = t.tucker(alg=0, rank=R, max_iter=500, rel_tol=1e-6,
+                r = t.tucker(alg=alg, rank=R, max_iter=500, rel_tol=1e-6,
                              delta=w["delta"], seed=0, precision=prec)
             except sp.SpError as e:
                 runs[name] = {"error": str(e)}
@@ -479,14 +558,14 @@ def main():
             runs[name] = {"seconds": r["seconds"], "iters": r["iters"],
                           "fp64_sweeps": r["fp64_sweeps"],
                           "final_error": float(r["errors"][-1])}
-            log("HOOI %s: %.3f s, %d iters, error %.12e" % (name, r["seconds"], r["iters"],
-                                                           r["errors"][-1]))
+            log("%s %s: %.3f s, %d iters, error %.12e" % (aname.upper(), name, r["seconds"],
+                                                         r["iters"], r["errors"][-1]))
         if "final_error" in runs.get("fp64", {}):
             e64 = runs["fp64"]["final_error"]
             for k, v in runs.items():
                 if "final_error" in v:
                     v["rel_fit_error_vs_fp64"] = abs(v["final_error"] - e64) / e64
-        out["hooi_time_to_fit"] = runs
+        out["%s_time_to_fit" % aname] = runs
 
     if not args.no_learned:
         del t
@@ -497,13 +576,22 @@ def main():
         log("cpu baseline (reference build on host cores)")
         out["cpu_baseline"] = cpu_baseline(inds, vals, N, n, R)
         if not args.no_hooi:
-            log("cpu HOOI baseline (oracle loop on host cores)")
-            cb = cpu_hooi_baseline(inds, vals, N, n, R, w["max_iter"], w["delta"],
-                                   s3ttmc_rate=out["cpu_baseline"].get("value"))
-            out["cpu_hooi_baseline"] = cb
+            log("cpu %s baseline (host cores)" % aname.upper())
+            rate = out["cpu_baseline"].get("value")
+            threads = out["cpu_baseline"].get("cores", 1)
+            if alg == 0:
+                cb = cpu_hooi_baseline(inds, vals, N, n, R, w["max_iter"], w["delta"],
+                                       s3ttmc_rate=rate)
+            elif rate and (level_bytes > 0.5 * host_memory_bytes() or nnz / rate > 20.0):
+                # one reference S3TTMc of the whole tensor does not fit host
+                # memory or the CPU time budget
+                cb = cpu_hoqri_estimate(N, n, nnz, R, rate, threads, level_bytes)
+            else:
+                cb = cpu_hoqri_baseline(inds, vals, N, n, R, w["max_iter"], w["delta"])
+            out["cpu_%s_baseline" % aname] = cb
             if "s_per_iter" in cb and hooi_fixed:
                 cb["gpu_fp64_speedup_per_iteration"] = cb["s_per_iter"] / hooi_fixed["s_per_iter"]
-                f64 = out.get("hooi_time_to_fit", {}).get("fp64", {})
+                f64 = out.get("%s_time_to_fit" % aname, {}).get("fp64", {})
                 if "iters" in f64:
                     # the CPU loop runs the same iteration sequence (same start,
                     # same stopping rule), so its time to fit is its
@@ -511,7 +599,7 @@ def main():
                     cb["time_to_fit_estimate_s"] = cb["s_per_iter"] * f64["iters"]
                     cb["time_to_fit_estimate_note"] = (
                         "CPU s/iteration x the %d iterations the FP64 run needed" % f64["iters"])
-                    for k, v in out["hooi_time_to_fit"].items():
+                    for k, v in out["%s_time_to_fit" % aname].items():
                         if "seconds" in v:
                             v["speedup_vs_cpu_time_to_fit"] = (cb["time_to_fit_estimate_s"] /
                                                                v["seconds"])

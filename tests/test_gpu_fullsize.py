@@ -1,7 +1,9 @@
 """S3TTMc at the benchmark sizes: the order-4 Zipf tensor (n = 10,000, 1M
 nnz, R = 10) against the oracle itself, and the order-6 planted tensor
-(n = 50,000, 5M nnz, R = 8) through size-independent properties (the oracle's
-opt5 level arrays would need hundreds of GB there).
+(n = 50,000, 5M nnz, R = 8) and the order-7 Zipf(1.2) tensor (n = 100,000,
+10M nnz, R = 6) through size-independent properties (the oracle's opt5 level
+arrays would need hundreds of GB there); the order-7 tensor's first 100,000
+nonzeros also against the oracle.
 
 Properties used:
 - homogeneity: Y(2U) = 2^(N-1) Y(U) bit for bit (scaling by a power of two
@@ -46,6 +48,31 @@ def planted6():
     return inds, vals
 
 
+@pytest.fixture(scope="module")
+def zipf7():
+    inds, vals = sp.generate(sp.GEN_ZIPF, 7, 100000, 10_000_000, 1.2, 1, seed=0)
+    return inds, vals
+
+
+SHAPES = {"zipf4": (10000, 4, 10), "planted6": (50000, 6, 8), "zipf7": (100000, 7, 6)}
+
+
+def test_zipf7_prefix_matches_oracle(zipf7):
+    inds, vals = zipf7
+    n, N, R = SHAPES["zipf7"]
+    k = 100_000
+    a, v = np.ascontiguousarray(inds[:k]), np.ascontiguousarray(vals[:k])
+    U = oracle.random_factor(0, n, R)
+    say("zipf7 prefix vs oracle", k)
+    t = sp.SpSymtensor(n, N, a, v)
+    Y = t.s3ttmc(U)
+    T = oracle.Tree(N, n, a, v)
+    Yo = T.s3ttmc(U, R, nthreads=16)
+    assert werr(Y, Yo, R, N) <= 1e-12
+    for l in range(N - 1):
+        assert t.level_size(l) == T.level_size(l)
+
+
 def test_zipf4_full_size_matches_oracle(zipf4):
     inds, vals = zipf4
     n, N, R = 10000, 4, 10
@@ -72,10 +99,10 @@ def unpack_core(core, R, N):
     return full.reshape((R,) * N)
 
 
-@pytest.mark.parametrize("which", ["zipf4", "planted6"])
+@pytest.mark.parametrize("which", ["zipf4", "planted6", "zipf7"])
 def test_full_size_properties(which, request):
     inds, vals = request.getfixturevalue(which)
-    n, N, R = (10000, 4, 10) if which == "zipf4" else (50000, 6, 8)
+    n, N, R = SHAPES[which]
     rng = np.random.default_rng(7)
     U = np.linalg.qr(rng.standard_normal((n, R)))[0]
     say(which, "properties: build")

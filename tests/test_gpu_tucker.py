@@ -44,6 +44,27 @@ def test_trajectory_matches_oracle(alg, ofn):
     assert np.allclose(r["U"].T @ r["U"], np.eye(R), atol=1e-10)
 
 
+def test_hoqri_order7_zipf_matches_oracle():
+    # the order-7 rank-6 shape of the HOQRI benchmark, at a size the oracle
+    # runs in seconds
+    N, n, R = 7, 400, 6
+    inds, vals = sp.generate(sp.GEN_ZIPF, N, n, 20000, 1.2, 1, seed=3)
+    say("hoqri order 7 zipf", n, len(vals), R)
+    t = sp.SpSymtensor(n, N, inds, vals)
+    r = t.tucker(alg=1, rank=R, max_iter=6, rel_tol=1e-14, delta=0.5, seed=0)
+    Uo, eo, _ = tucker.hoqri(oracle.Tree(N, n, inds, vals), R, 6, rel_tol=1e-14, delta=0.5,
+                             seed=0)
+    assert r["iters"] == len(eo)
+    assert np.allclose(r["errors"], eo, rtol=0, atol=1e-10)
+    assert tucker.subspace_cos(r["U"], Uo) > 1 - 1e-9
+    # the mixed forms reach the FP64 fit under the reference stopping rule
+    e64 = t.tucker(alg=1, rank=R, max_iter=200, rel_tol=1e-6, delta=0.5, seed=0)["errors"][-1]
+    for prec in (sp.PREC_FP32, sp.PREC_BF16):
+        e = t.tucker(alg=1, rank=R, max_iter=200, rel_tol=1e-6, delta=0.5, seed=0,
+                     precision=prec)["errors"][-1]
+        assert abs(e - e64) / e64 <= 1e-6
+
+
 def test_exact_rank_recovered():
     tt, v, X, U0 = exact_rank_tensor(n=9, R=2, N=4, seed=1)
     say("tucker exact rank")
