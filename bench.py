@@ -354,6 +354,8 @@ def learned_study(dim, nnz, pin, ntrain, ntest, iters, steps):
         feats.append(F)
         facs.append(Us[-1])
         del t
+        if s % 10 == 0 or s == ntrain:
+            log("learned: teacher run %d/%d, %.1f s" % (s, ntrain, time.perf_counter() - t0))
     t_teacher = time.perf_counter() - t0
     model = learned.WarmStartModel(R, seed=0)
     t0 = time.perf_counter()
