@@ -42,22 +42,49 @@ __global__ void sketch_init(u32 n, u64 seed, double *__restrict__ s) {
   s[t] = sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
 }
 
-// One propagation round over the rem-sorted owner lists: thread block per
-// node r, threads over the 62 sketch dimensions; then L2 row normalisation.
-__global__ void __launch_bounds__(64)
-sketch_round(const u64 *__restrict__ segptr, const u32 *__restrict__ tup,
+// One propagation round over the rem-sorted owner lists, in two passes so
+// that a hub row (millions of owners in a Zipf tensor) is spread over many
+// workgroups:
+//   sketch_piece:  workgroup per piece (at most kSketchPiece owners of one
+//                  row); wave w takes owners w, w+4, ...; the 4 wave sums are
+//                  added in a fixed order into the piece's partial row;
+//   sketch_finish: workgroup per row, sums its pieces' partials in piece
+//                  order, then L2-normalises the row.
+constexpr u64 kSketchPiece = 2048;
+
+struct SketchPiece {
+  u64 o0, o1;
+};
+
+__global__ void __launch_bounds__(256)
+sketch_piece(const SketchPiece *__restrict__ pieces, const u32 *__restrict__ tup,
              const double *__restrict__ val, int m, const double *__restrict__ s,
-             double *__restrict__ out) {
-  const u32 r = blockIdx.x;
-  const int k = threadIdx.x;
+             double *__restrict__ partial) {
+  __shared__ double red[4][64];
+  const SketchPiece pc = pieces[blockIdx.x];
+  const int k = threadIdx.x & 63, w = threadIdx.x >> 6;
   double acc = 0.0;
   if (k < SKETCH) {
-    for (u64 o = segptr[r]; o < segptr[r + 1]; ++o) {
+    for (u64 o = pc.o0 + w; o < pc.o1; o += 4) {
       double sum = 0.0;
       for (int a = 0; a < m; ++a) sum += s[(u64)tup[o * m + a] * SKETCH + k];
       acc += val[o] * sum;
     }
   }
+  red[w][k] = acc;
+  __syncthreads();
+  if (w == 0 && k < SKETCH)
+    partial[(u64)blockIdx.x * SKETCH + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+}
+
+__global__ void __launch_bounds__(64)
+sketch_finish(const u64 *__restrict__ piece_ptr, const double *__restrict__ partial,
+              double *__restrict__ out) {
+  const u32 r = blockIdx.x;
+  const int k = threadIdx.x;
+  double acc = 0.0;
+  if (k < SKETCH)
+    for (u64 p = piece_ptr[r]; p < piece_ptr[r + 1]; ++p) acc += partial[p * SKETCH + k];
   double sq = acc * acc;
 #pragma unroll
   for (int w = 32; w > 0; w >>= 1) sq += __shfl_xor(sq, w, 64);
@@ -441,13 +468,40 @@ void node_features(Ctx &c, int rounds, u64 seed, float *f) {
   DevBuf s0((u64)n * SKETCH * sizeof(double)), s1((u64)n * SKETCH * sizeof(double));
   sketch_init<<<grid_for((u64)n * SKETCH, 256), 256, 0, c.stream>>>(n, seed, s0.as<double>());
   SP_CHECK_LAUNCH();
+  // pieces of at most kSketchPiece owners, row by row
+  std::vector<SketchPiece> pieces;
+  std::vector<u64> piece_ptr(n + 1);
+  const auto &sp = c.h_segptr;
+  for (u32 r = 0; r < n; ++r) {
+    piece_ptr[r] = pieces.size();
+    for (u64 o = sp[r]; o < sp[r + 1]; o += kSketchPiece)
+      pieces.push_back(SketchPiece{o, std::min(sp[r + 1], o + kSketchPiece)});
+  }
+  piece_ptr[n] = pieces.size();
+  const u64 np = pieces.size();
+  if (np >= (1ull << 31)) throw Error(SP_ERR_UNSUPPORTED, "too many sketch pieces");
+  DevBuf dpieces(std::max<u64>(1, np) * sizeof(SketchPiece));
+  DevBuf dpp((u64)(n + 1) * sizeof(u64));
+  DevBuf partial(std::max<u64>(1, np) * SKETCH * sizeof(double));
+  if (np)
+    SP_HIP(hipMemcpyAsync(dpieces.p, pieces.data(), np * sizeof(SketchPiece),
+                          hipMemcpyHostToDevice, c.stream));
+  SP_HIP(hipMemcpyAsync(dpp.p, piece_ptr.data(), (u64)(n + 1) * sizeof(u64),
+                        hipMemcpyHostToDevice, c.stream));
   for (int r = 0; r < rounds; ++r) {
-    sketch_round<<<n, 64, 0, c.stream>>>(c.w_segptr.as<u64>(), c.w_tuple.as<u32>(),
-                                         c.w_val.as<double>(), m, s0.as<double>(),
-                                         s1.as<double>());
+    if (np) {
+      sketch_piece<<<(unsigned)np, 256, 0, c.stream>>>(
+          dpieces.as<SketchPiece>(), c.w_tuple.as<u32>(), c.w_val.as<double>(), m,
+          s0.as<double>(), partial.as<double>());
+      SP_CHECK_LAUNCH();
+    }
+    sketch_finish<<<n, 64, 0, c.stream>>>(dpp.as<u64>(), partial.as<double>(),
+                                          s1.as<double>());
     SP_CHECK_LAUNCH();
     std::swap(s0, s1);
   }
+  // the host vectors above are read by the async copies: finish them here
+  SP_HIP(hipStreamSynchronize(c.stream));
   DevBuf partners((u64)n * sizeof(unsigned int));
   SP_HIP(hipMemsetAsync(partners.p, 0, (u64)n * sizeof(unsigned int), c.stream));
   if (c.depth >= 2) {

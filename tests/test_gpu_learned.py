@@ -26,6 +26,20 @@ def test_features_match_oracle():
     assert np.allclose(F, Fo, rtol=1e-5, atol=1e-6)
 
 
+def test_features_hub_rows_split_over_pieces():
+    # Zipf(1.2): node 1 owns tens of thousands of owners, so its sketch row
+    # is summed over many pieces
+    inds, vals = sp.generate(sp.GEN_ZIPF, 5, 300, 30000, 1.2, 1, seed=4)
+    say("features zipf hubs")
+    t = sp.SpSymtensor(300, 5, inds, vals)
+    F = learned.features(t, rounds=3, seed=2)
+    Fo = mlp.node_features(5, 300, inds, vals, rounds=3, seed=2)
+    deg = np.expm1(Fo[:, 0])
+    assert deg.max() > 8 * 2048
+    assert np.allclose(F, Fo, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(learned.features(t, rounds=3, seed=2), F)
+
+
 def test_loss_and_gradient_match_oracle():
     rng = np.random.default_rng(0)
     din, dh, dout, N = 64, 256, 8, 300
