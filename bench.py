@@ -412,6 +412,90 @@ def learned_study(dim, nnz, pin, ntrain, ntest, iters, steps):
             "held_out": {k: summary(rows[k]) for k in starts}}
 
 
+def learned_study_hoqri(nnz, ntrain, ntest, steps):
+    """Learned warm start for TuckerHOQRI on seeded order-7 Zipf(1.2) tensors
+    (n = 100,000, U(0,1) values, rank 6, delta 0.5).
+
+    Training targets: the factors plain HOQRI converges to from the
+    reference's random start on `ntrain` seeds.  Held-out seeds run HOQRI
+    under the reference stopping rule |error - previous| < 1e-6 from the
+    random start, the hypergraph sketch start and the learned start; each
+    reports iterations, final error, its fit error relative to the
+    random-start run, and time including the cost of producing the start."""
+    from symprop_amd import learned
+    N, n, R, delta = 7, 100000, 6, 0.5
+
+    def tensor(seed):
+        inds, vals = sp.generate(sp.GEN_ZIPF, N, n, nnz, 1.2, 1, seed=seed)
+        t = sp.SpSymtensor(n, N, inds, vals)
+        t.sync()
+        return t
+
+    def run(t, U0):
+        kw = dict(alg=1, rank=R, max_iter=500, rel_tol=1e-6, delta=delta, seed=0)
+        return t.tucker(U0=U0, **kw) if U0 is not None else t.tucker(**kw)
+
+    feats, facs, teacher_its = [], [], []
+    t0 = time.perf_counter()
+    for s in range(1, ntrain + 1):
+        t = tensor(s)
+        r = run(t, None)
+        teacher_its.append(r["iters"])
+        feats.append(learned.features(t))
+        facs.append(r["U"])
+        del t
+        log("learned hoqri: teacher run %d/%d (%d iterations), %.1f s"
+            % (s, ntrain, r["iters"], time.perf_counter() - t0))
+    t_teacher = time.perf_counter() - t0
+    model = learned.WarmStartModel(R, seed=0)
+    t0 = time.perf_counter()
+    losses = model.fit(feats, facs, steps=steps, batch=4096, lr=1e-3)
+    t_fit = time.perf_counter() - t0
+    del feats, facs
+    log("learned hoqri: fit %.1f s, loss %.4f -> %.4f"
+        % (t_fit, float(losses[:5].mean()), float(losses[-10:].mean())))
+    starts = ("random", "sketch", "learned")
+    rows = {k: [] for k in starts}
+    for s in range(10001, 10001 + ntest):
+        t = tensor(s)
+        e_rand = None
+        for k in starts:
+            t0 = time.perf_counter()
+            U0 = None
+            if k != "random":
+                F = learned.features(t)
+                U0 = learned.spectral_start(F, R) if k == "sketch" else model.predict(F)
+            t_init = time.perf_counter() - t0
+            r = run(t, U0)
+            e = float(r["errors"][-1])
+            if k == "random":
+                e_rand = e
+            rows[k].append({"seed": s, "iters": r["iters"], "final_error": e,
+                            "rel_fit_error_vs_random": abs(e - e_rand) / e_rand,
+                            "init_s": t_init, "seconds": t_init + r["seconds"]})
+        log("learned hoqri seed %d: " % s + ", ".join(
+            "%s %d it err %.10f" % (k, rows[k][-1]["iters"], rows[k][-1]["final_error"])
+            for k in starts))
+        del t
+
+    def summary(rs):
+        return {"median_iters": float(np.median([r["iters"] for r in rs])),
+                "mean_final_error": float(np.mean([r["final_error"] for r in rs])),
+                "max_rel_fit_error_vs_random": float(max(r["rel_fit_error_vs_random"]
+                                                         for r in rs)),
+                "median_seconds": float(np.median([r["seconds"] for r in rs])),
+                "runs": rs}
+
+    return {"config": {"order": N, "dim": n, "nnz": nnz, "zipf_s": 1.2, "rank": R,
+                       "delta": delta, "algorithm": "HOQRI", "train_tensors": ntrain,
+                       "held_out": ntest, "criterion": "|error - previous| < 1e-6"},
+            "train": {"teacher": "HOQRI from the reference's random start",
+                      "teacher_iters": teacher_its, "teacher_s": t_teacher, "fit_s": t_fit,
+                      "steps": steps, "loss_first": float(losses[:5].mean()),
+                      "loss_last": float(losses[-10:].mean())},
+            "held_out": {k: summary(rows[k]) for k in starts}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,6 +513,11 @@ def main():
     ap.add_argument("--learned-test", type=int, default=4)
     ap.add_argument("--learned-iters", type=int, default=40)
     ap.add_argument("--learned-steps", type=int, default=800)
+    ap.add_argument("--learned-hoqri-only", action="store_true",
+                    help="only run the HOQRI learned warm-start study on order-7 Zipf tensors")
+    ap.add_argument("--learned-hoqri-nnz", type=int, default=10_000_000)
+    ap.add_argument("--learned-hoqri-train", type=int, default=16)
+    ap.add_argument("--learned-hoqri-test", type=int, default=4)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(reference for HBM counter scaling) and exit")
@@ -451,6 +540,11 @@ def main():
                     args.learned_test, args.learned_iters, args.learned_steps)
     if args.learned_only:
         print(json.dumps({"learned_warm_start": learned_study(*learned_args)}), flush=True)
+        return
+    if args.learned_hoqri_only:
+        print(json.dumps({"learned_warm_start_hoqri": learned_study_hoqri(
+            args.learned_hoqri_nnz, args.learned_hoqri_train, args.learned_hoqri_test,
+            args.learned_steps)}), flush=True)
         return
 
     w = workload(args.workload)
