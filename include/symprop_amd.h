@@ -175,6 +175,16 @@ int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
 int sp_hoevd_gram(sp_ctx *ctx, double *A);
 int sp_hoevd_init(sp_ctx *ctx, uint32_t R, double *U, double *eigvals);
 
+/* Eigenvectors of the `rank` largest eigenvalues of the symmetric dim x dim
+ * matrix A (host, row-major; only its values are read): U (dim x rank,
+ * row-major) with columns in ascending eigenvalue order and w (optional,
+ * rank values, ascending) -- matrix::syevd, src/dense/matrixops.cpp:77-100
+ * (dsyevd, then columns dim-rank..dim-1), on the context's device with
+ * rocSOLVER dsyevd (dsyevj when SP_EIG=syevj).  Eigenvector signs are
+ * arbitrary, as with LAPACK. */
+int sp_syevd_top(sp_ctx *ctx, const double *A, uint64_t dim, uint32_t rank, double *U,
+                 double *w);
+
 /* ---- Learned warm start.  No counterpart in the reference: the predicted
  * factor enters at its initialisation hook (hosvd_init,
  * src/sparse/spsymtensor.cpp:339-383) through sp_tucker_run's U0.

@@ -68,6 +68,7 @@ SIGNATURES = {
     "sp_ctx_info": (ctypes.c_int, [vp, u32p, u32p, u64p]),
     "sp_hoevd_gram": (ctypes.c_int, [vp, f64p]),
     "sp_hoevd_init": (ctypes.c_int, [vp, ctypes.c_uint32, f64p, f64p]),
+    "sp_syevd_top": (ctypes.c_int, [vp, f64p, ctypes.c_uint64, ctypes.c_uint32, f64p, f64p]),
     "sp_node_features": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint64, f32p]),
     "sp_mlp_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(vp)]),
@@ -291,6 +292,17 @@ class SpSymtensor:
         w = np.zeros(rank)
         _check(lib().sp_hoevd_init(self.h, rank, _p(U, f64p), _p(w, f64p)))
         self.R = rank
+        return U, w
+
+    def syevd_top(self, A, rank):
+        """Eigenvectors of the `rank` largest eigenvalues of the symmetric
+        matrix A (matrix::syevd): (U, w), columns and values ascending."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        n = A.shape[0]
+        assert A.shape == (n, n)
+        U = np.zeros((n, rank))
+        w = np.zeros(rank)
+        _check(lib().sp_syevd_top(self.h, _p(A, f64p), n, rank, _p(U, f64p), _p(w, f64p)))
         return U, w
 
     # decomposition -------------------------------------------------------

@@ -553,34 +553,42 @@ void dense_gram(Ctx &c) {
   SP_HIP(hipStreamSynchronize(c.stream));
 }
 
-void dense_eig_full(Ctx &c, double *D, double *E, double *resid, rocblas_int *info);
+// All eigenpairs of the P x P symmetric G by rocSOLVER (syevd, or syevj when
+// selected): eigenvectors overwrite G column-major in ascending order,
+// values in D.
+void dense_eig_full(Ctx &c, double *G, u64 P, double *D, double *E, double *resid,
+                    rocblas_int *info);
+
+void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta) {
+  if (R > P) throw Error(SP_ERR_ARG, "rank exceeds the matrix order");
+  // workspace: D (P), E (P), residual (2), then int words
+  c.ws_eig.reserve((2 * P + 2) * sizeof(double) + 4 * sizeof(rocblas_int));
+  double *D = c.ws_eig.as<double>();
+  double *E = D + P;
+  double *resid = E + P;
+  rocblas_int *info = reinterpret_cast<rocblas_int *>(resid + 2);
+  dense_eig_full(c, G, P, D, E, resid, info);
+  std::vector<double> lam(P);
+  SP_HIP(hipMemcpyAsync(lam.data(), D, P * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  theta.resize(R);
+  for (u32 r = 0; r < R; ++r) theta[r] = lam[P - 1 - r];
+  top_columns<<<grid_for(P * R, 256), 256, 0, c.stream>>>(G, P, R, V);
+  SP_CHECK_LAUNCH();
+}
 
 void dense_leftsvd_y(Ctx &c, double *sigma) {
   dense_gram(c);
   const u64 P = c.P, n = c.dim;
   const u32 R = c.R;
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds packed width");
-  // workspace: D (P), E (P), residual (1), theta (R), then int words
-  c.ws_eig.reserve((2 * P + 2 + R) * sizeof(double) + 4 * sizeof(rocblas_int));
-  double *D = c.ws_eig.as<double>();
-  double *E = D + P;
-  double *resid = E + P;
-  double *theta_d = resid + 2;
-  rocblas_int *info = reinterpret_cast<rocblas_int *>(theta_d + R);
   // top-R eigenvectors of G as V (P x R, row-major), eigenvalues descending
   DevBuf &V = c.ws_pr;
   V.reserve(P * R * sizeof(double));
-  std::vector<double> theta(R);
-  {
-    dense_eig_full(c, D, E, resid, info);
-    std::vector<double> lam(P);
-    SP_HIP(hipMemcpyAsync(lam.data(), D, P * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    SP_HIP(hipStreamSynchronize(c.stream));
-    for (u32 r = 0; r < R; ++r) theta[r] = lam[P - 1 - r];
-    top_columns<<<grid_for(P * R, 256), 256, 0, c.stream>>>(c.d_G.as<double>(), P, R,
-                                                           V.as<double>());
-    SP_CHECK_LAUNCH();
-  }
+  std::vector<double> theta;
+  eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+  c.ws_eig.reserve((2 * P + 2 + R) * sizeof(double) + 4 * sizeof(rocblas_int));
+  double *theta_d = c.ws_eig.as<double>();
   if (sigma)
     for (u32 r = 0; r < R; ++r) sigma[r] = std::sqrt(std::max(theta[r], 0.0));
   // The Gram's eigenvalues carry absolute errors of about eps * lambda_max,
@@ -607,23 +615,20 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   SP_HIP(hipStreamSynchronize(c.stream));
 }
 
-// All eigenpairs of G by rocSOLVER (syevd, or syevj when selected):
-// eigenvectors overwrite G column-major in ascending order, values in D.
-void dense_eig_full(Ctx &c, double *D, double *E, double *resid, rocblas_int *info) {
-  const u64 P = c.P;
+void dense_eig_full(Ctx &c, double *G, u64 P, double *D, double *E, double *resid,
+                    rocblas_int *info) {
   rocblas_handle h = handle_for(c);
   if (c.eig_method == 1) {
     ScopedTimer t(c, "syevj");
     rocblas_status st = rocsolver_dsyevj(h, rocblas_esort_ascending, rocblas_evect_original,
                                          rocblas_fill_upper, (rocblas_int)P,
-                                         c.d_G.as<double>(), (rocblas_int)P, 0.0, resid, 100,
+                                         G, (rocblas_int)P, 0.0, resid, 100,
                                          info + 1, D, info);
     if (st != rocblas_status_success) throw Error(SP_ERR_DEVICE, "rocsolver_dsyevj failed");
   } else {
     ScopedTimer t(c, "syevd");
     rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper,
-                                         (rocblas_int)P, c.d_G.as<double>(),
-                                         (rocblas_int)P, D, E, info);
+                                         (rocblas_int)P, G, (rocblas_int)P, D, E, info);
     if (st != rocblas_status_success) throw Error(SP_ERR_DEVICE, "rocsolver_dsyevd failed");
   }
   rocblas_int hinfo = 0;

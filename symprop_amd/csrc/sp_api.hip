@@ -356,6 +356,30 @@ int sp_hoevd_init(sp_ctx *ctx, uint32_t R, double *U, double *eigvals) {
   return sp_factor_upload(ctx, Uh.data(), R);
 }
 
+int sp_syevd_top(sp_ctx *ctx, const double *A, uint64_t dim, uint32_t rank, double *U,
+                 double *w) {
+  return guard(ctx, [&] {
+    need(ctx && A && U, SP_ERR_ARG, "null argument");
+    need(dim >= 1 && dim <= 65535, SP_ERR_ARG, "dim must be in [1,65535]");
+    need(rank >= 1 && rank <= dim, SP_ERR_ARG, "rank must be in [1,dim]");
+    Ctx &c = ctx->c;
+    c.ws_sym.reserve(dim * dim * sizeof(double));
+    c.ws_pr.reserve(dim * rank * sizeof(double));
+    SP_HIP(hipMemcpyAsync(c.ws_sym.p, A, dim * dim * sizeof(double), hipMemcpyHostToDevice,
+                          c.stream));
+    std::vector<double> theta, V(dim * rank);
+    eig_top(c, c.ws_sym.as<double>(), dim, rank, c.ws_pr.as<double>(), theta);
+    SP_HIP(hipMemcpyAsync(V.data(), c.ws_pr.p, dim * rank * sizeof(double),
+                          hipMemcpyDeviceToHost, c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+    // ascending order, as matrix::syevd returns the columns dim-rank..dim-1
+    for (u64 i = 0; i < dim; ++i)
+      for (u32 j = 0; j < rank; ++j) U[i * rank + j] = V[i * rank + (rank - 1 - j)];
+    if (w)
+      for (u32 j = 0; j < rank; ++j) w[j] = theta[rank - 1 - j];
+  });
+}
+
 int sp_node_features(sp_ctx *ctx, uint32_t rounds, uint64_t seed, float *out) {
   return guard(ctx, [&] {
     need(ctx && out, SP_ERR_ARG, "null argument");

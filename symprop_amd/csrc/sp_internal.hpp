@@ -174,6 +174,7 @@ struct Ctx {
   DevBuf ws_eig;   // 2P + R doubles + info words
   DevBuf ws_tiles; // Gram tile list
   int eig_method = 0;  // 0: rocSOLVER syevd (divide and conquer), 1: syevj (Jacobi)
+  DevBuf ws_sym;   // symmetric matrix uploaded by sp_syevd_top
 
   // chunking of the work list for the current R: plan[0] with 64-owner
   // batches (generic, FP32 and BF16 kernels), plan[1] with the FP64 fast
@@ -235,6 +236,11 @@ double dense_core_norm(Ctx &c);
 // HOEVD initial factor (hoevd.hip)
 void hoevd_gram(Ctx &c, double *A);
 void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out);
+
+// Top-R eigenpairs of the P x P symmetric matrix G (device, destroyed) by
+// rocSOLVER (c.eig_method): V (device, P x R row-major) and theta (host),
+// descending.
+void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta);
 
 // learned warm start (learned.hip)
 struct Mlp;

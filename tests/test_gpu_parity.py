@@ -263,6 +263,48 @@ def test_qr_well_and_ill_conditioned(cond):
     assert np.allclose(np.abs(Q.T @ Qref), np.eye(R), atol=1e-5 if cond > 1e6 else 1e-10)
 
 
+def spectrum_matrix(P, lam, seed):
+    Q = np.linalg.qr(np.random.default_rng(seed).standard_normal((P, P)))[0]
+    return (Q * lam) @ Q.T
+
+
+@pytest.mark.parametrize("eig", ["syevd", "syevj"])
+@pytest.mark.parametrize("P,R,kind", [(1, 1, "random"), (2, 1, "random"), (3, 2, "random"),
+                                      (50, 8, "random"), (220, 10, "gram"),
+                                      (462, 6, "clusters"), (300, 8, "rank3")])
+def test_syevd_top_matches_numpy(P, R, kind, eig, monkeypatch):
+    # matrix::syevd: top-R eigenvectors, ascending, against numpy's LAPACK
+    monkeypatch.setenv("SP_EIG", eig)
+    rng = np.random.default_rng(P + R)
+    if kind == "random":
+        A = rng.standard_normal((P, P))
+        A = A + A.T
+    elif kind == "gram":
+        # Gram-like spectrum: fast decay, then a flat tail
+        lam = np.sort(np.concatenate([1e5 * 0.5 ** np.arange(R + 4), rng.random(P - R - 4)]))
+        A = spectrum_matrix(P, lam, P)
+    elif kind == "clusters":
+        # tight clusters inside the top R, a gap of 2x below them
+        top = np.repeat([10.0, 7.0, 7.0 + 1e-9], [2, 2, 2])
+        lam = np.sort(np.concatenate([top, 3.0 * rng.random(P - R)]))
+        A = spectrum_matrix(P, lam, P)
+    else:  # rank 3: zero eigenvalues inside the wanted set
+        B = rng.standard_normal((P, 3))
+        A = B @ B.T
+    say("syevd_top", P, R, kind, eig)
+    t = sp.SpSymtensor(4, 3, np.array([[1, 2, 3]], dtype=np.uint32), np.ones(1))
+    U, w = t.syevd_top(A, R)
+    lam, V = np.linalg.eigh(A)
+    scale = np.abs(lam).max()
+    assert np.allclose(w, lam[P - R:], rtol=0, atol=1e-12 * scale)
+    assert np.allclose(U.T @ U, np.eye(R), atol=1e-11)
+    assert np.abs(A @ U - U * w).max() <= 1e-11 * scale
+    if kind != "rank3":
+        # the wanted subspace (separated from the rest by a gap)
+        c = np.linalg.svd(U.T @ V[:, P - R:], compute_uv=False)
+        assert c.min() > 1 - 1e-10
+
+
 @pytest.mark.parametrize("eig", ["syevd", "syevj"])
 def test_leftsvd_subspace(eig, monkeypatch):
     monkeypatch.setenv("SP_EIG", eig)
