@@ -510,17 +510,23 @@ struct FastEntry {
   }
 
 const FastEntry kFast[] = {
-    // order 3
-    SP_FAST(2, 2, 1), SP_FAST(2, 4, 1), SP_FAST(2, 8, 1), SP_FAST(2, 10, 1),
-    // order 4
-    SP_FAST(3, 2, 2), SP_FAST(3, 3, 2), SP_FAST(3, 4, 2), SP_FAST(3, 8, 2),
-    SP_FAST(3, 10, 2),
-    // order 5
-    SP_FAST(4, 2, 2), SP_FAST(4, 3, 2), SP_FAST(4, 4, 2), SP_FAST(4, 8, 2),
-    // order 6
-    SP_FAST(5, 3, 3), SP_FAST(5, 4, 3), SP_FAST(5, 8, 3),
-    // order 7
-    SP_FAST(6, 3, 3), SP_FAST(6, 6, 3),
+    // order 3, R = 2..16
+    SP_FAST(2, 2, 1), SP_FAST(2, 3, 1), SP_FAST(2, 4, 1), SP_FAST(2, 5, 1), SP_FAST(2, 6, 1),
+    SP_FAST(2, 7, 1), SP_FAST(2, 8, 1), SP_FAST(2, 9, 1), SP_FAST(2, 10, 1), SP_FAST(2, 11, 1),
+    SP_FAST(2, 12, 1), SP_FAST(2, 13, 1), SP_FAST(2, 14, 1), SP_FAST(2, 15, 1), SP_FAST(2, 16, 1),
+    // order 4, R = 2..16
+    SP_FAST(3, 2, 2), SP_FAST(3, 3, 2), SP_FAST(3, 4, 2), SP_FAST(3, 5, 2), SP_FAST(3, 6, 2),
+    SP_FAST(3, 7, 2), SP_FAST(3, 8, 2), SP_FAST(3, 9, 2), SP_FAST(3, 10, 2), SP_FAST(3, 11, 2),
+    SP_FAST(3, 12, 2), SP_FAST(3, 13, 2), SP_FAST(3, 14, 2), SP_FAST(3, 15, 2), SP_FAST(3, 16, 2),
+    // order 5, R = 2..12
+    SP_FAST(4, 2, 2), SP_FAST(4, 3, 2), SP_FAST(4, 4, 2), SP_FAST(4, 5, 2), SP_FAST(4, 6, 2),
+    SP_FAST(4, 7, 2), SP_FAST(4, 8, 2), SP_FAST(4, 9, 2), SP_FAST(4, 10, 2), SP_FAST(4, 11, 2),
+    SP_FAST(4, 12, 2),
+    // order 6, R = 2..8
+    SP_FAST(5, 2, 3), SP_FAST(5, 3, 3), SP_FAST(5, 4, 3), SP_FAST(5, 5, 3), SP_FAST(5, 6, 3),
+    SP_FAST(5, 7, 3), SP_FAST(5, 8, 3),
+    // order 7, R = 2..6
+    SP_FAST(6, 2, 3), SP_FAST(6, 3, 3), SP_FAST(6, 4, 3), SP_FAST(6, 5, 3), SP_FAST(6, 6, 3),
 };
 
 }  // namespace

@@ -139,6 +139,31 @@ def test_s3ttmc_random_vs_oracle(N, n, cnt, R, seed):
         assert werr(Y, Yd, R, N) <= 1e-11
 
 
+FAST_SHAPES = ([(3, R) for R in range(2, 17)] + [(4, R) for R in range(2, 17)] +
+               [(5, R) for R in range(2, 13)] + [(6, R) for R in range(2, 9)] +
+               [(7, R) for R in range(2, 7)])
+
+
+@pytest.mark.parametrize("N,R", FAST_SHAPES)
+def test_s3ttmc_fast_kernel_every_instantiated_shape(N, R):
+    # every (order, rank) with a compiled MFMA kernel: FP64 against the
+    # oracle, the FP32 and BF16 forms against FP64, and the MFMA kernels ran
+    n = {3: 200, 4: 120, 5: 60, 6: 40, 7: 25}[N]
+    inds, vals = sp.generate(sp.GEN_ZIPF, N, n, 3000, 1.1, 1, seed=N * 100 + R)
+    U = np.linalg.qr(np.random.default_rng(R).standard_normal((n, R)))[0]
+    say("s3ttmc fast shape", N, R, len(vals))
+    t = sp.SpSymtensor(n, N, inds, vals)
+    t.timing(True)
+    Y = t.s3ttmc(U)
+    assert t.timing_get("s3ttmc_mfma64")[1] == 1
+    Yo = oracle.Tree(N, n, inds, vals).s3ttmc(U, R, nthreads=4)
+    assert werr(Y, Yo, R, N) <= 1e-12
+    assert werr(t.s3ttmc(U, precision=sp.PREC_FP32), Y, R, N) < 2e-5
+    assert werr(t.s3ttmc(U, precision=sp.PREC_BF16), Y, R, N) < 1e-2
+    assert t.timing_get("s3ttmc_mfma_f32")[1] == 1
+    assert t.timing_get("s3ttmc_mfma_bf16")[1] == 1
+
+
 def test_s3ttmc_hub_rows_split():
     # Zipf tensor: hub rows carry far more owners than one work chunk
     inds, vals = sp.generate(sp.GEN_ZIPF, 4, 2000, 60000, 1.1, 1, seed=3)
