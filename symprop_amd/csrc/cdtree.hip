@@ -113,18 +113,40 @@ __device__ inline u64 lower_bound_key(const K *__restrict__ a, u64 n, K x) {
 // CSR from parents to children: the children of parent key k are the child
 // keys in [k << b, (k+1) << b), so ptr[i] = lower_bound(children, key_i << b)
 // (children with smaller prefixes sort first).  ptr[np] = nc.
+// The search runs within the children sharing the parent's leading field
+// (cfidx: first_field_index of the child level; sp = parent fields).
 template <class KC, class KP>
 __global__ void child_ptr(const KC *__restrict__ child, u64 nc, int b,
-                          const KP *__restrict__ parent, u64 np, u64 *__restrict__ ptr) {
+                          const KP *__restrict__ parent, u64 np, int sp,
+                          const u64 *__restrict__ cfidx, u64 *__restrict__ ptr) {
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i > np) return;
-  ptr[i] = (i == np) ? nc : lower_bound_key<KC>(child, nc, (KC)parent[i] << b);
+  if (i == np) {
+    ptr[i] = nc;
+    return;
+  }
+  const KP key = parent[i];
+  const u32 f = (u32)(key >> (b * (sp - 1)));
+  const u64 lo = cfidx[f];
+  ptr[i] = lo + lower_bound_key<KC>(child + lo, cfidx[f + 1] - lo, (KC)key << b);
 }
 
-// dep_list: node j (s fields) minus position k, searched in the level below.
+// idx[v] = first key of the level whose leading field is >= v, v = 0..dim+1
+// (a level's keys hold fields of b bits, indices 1..dim, leading field most
+// significant); idx[dim+1] = n.
+template <class K>
+__global__ void first_field_index(const K *__restrict__ keys, u64 n, int fields, int b,
+                                  u32 dim, u64 *__restrict__ idx) {
+  const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > (u64)dim + 1) return;
+  idx[v] = v > dim ? n : lower_bound_key<K>(keys, n, (K)v << (b * (fields - 1)));
+}
+
+// dep_list: node j (s fields) minus position k, searched in the level below
+// within the keys that share its leading field (fidx of the level below).
 template <class KC, class KP>
 __global__ void exclude_one(const KC *__restrict__ keys, u64 n, int s, int b,
-                            const KP *__restrict__ below, u64 nb,
+                            const KP *__restrict__ below, const u64 *__restrict__ fidx,
                             u64 *__restrict__ dep) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * (u64)s) return;
@@ -136,7 +158,9 @@ __global__ void exclude_one(const KC *__restrict__ keys, u64 n, int s, int b,
   // fields 0..k-1; for k == 0 the shift would equal the key width
   const KC hi = (k == 0) ? (KC)0 : (key >> (b * (s - k)));
   const KC sub = (hi << lo_bits) | (key & lo_mask);
-  dep[t] = lower_bound_key<KP>(below, nb, (KP)sub);
+  const u32 f = (u32)(sub >> (b * (s - 2)));
+  const u64 lo = fidx[f];
+  dep[t] = lo + lower_bound_key<KP>(below + lo, fidx[f + 1] - lo, (KP)sub);
 }
 
 __global__ void owner_fields(const u32 *__restrict__ rec, u64 no, int N,
@@ -346,15 +370,23 @@ void link_levels(Ctx &c, int l) {
   // CSR from level l (parents, KP) to level l+1 (children, KC)
   DevLevel &P = c.levels[l];
   DevLevel &C = c.levels[l + 1];
+  DevBuf fidx(((u64)c.dim + 2) * sizeof(u64)), cfidx(((u64)c.dim + 2) * sizeof(u64));
+  first_field_index<KP><<<grid_for((u64)c.dim + 2, BLK), BLK, 0, c.stream>>>(
+      P.keys.as<KP>(), P.size, l + 1, c.bits, c.dim, fidx.as<u64>());
+  SP_CHECK_LAUNCH();
+  first_field_index<KC><<<grid_for((u64)c.dim + 2, BLK), BLK, 0, c.stream>>>(
+      C.keys.as<KC>(), C.size, l + 2, c.bits, c.dim, cfidx.as<u64>());
+  SP_CHECK_LAUNCH();
   P.level_ptr.alloc((P.size + 1) * sizeof(u64));
   child_ptr<KC, KP><<<grid_for(P.size + 1, BLK), BLK, 0, c.stream>>>(
-      C.keys.as<KC>(), C.size, c.bits, P.keys.as<KP>(), P.size, P.level_ptr.as<u64>());
+      C.keys.as<KC>(), C.size, c.bits, P.keys.as<KP>(), P.size, l + 1, cfidx.as<u64>(),
+      P.level_ptr.as<u64>());
   SP_CHECK_LAUNCH();
   // dep_list of level l+1 against level l
   const int s = l + 2;
   C.dep_list.alloc(C.size * (u64)s * sizeof(u64));
   exclude_one<KC, KP><<<grid_for(C.size * (u64)s, BLK), BLK, 0, c.stream>>>(
-      C.keys.as<KC>(), C.size, s, c.bits, P.keys.as<KP>(), P.size,
+      C.keys.as<KC>(), C.size, s, c.bits, P.keys.as<KP>(), fidx.as<u64>(),
       C.dep_list.as<u64>());
   SP_CHECK_LAUNCH();
   SP_HIP(hipStreamSynchronize(c.stream));
