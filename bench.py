@@ -278,6 +278,13 @@ def dense_y_side(t, N, n, R, delta, iters=3):
     G = diag(sqrt W) Y^T Y diag(sqrt W) (64 x 64 upper tiles, FP64 MFMA),
     U^T Y and Y M^T."""
     P = binom(R + N - 2, N - 1)
+    # one untimed iteration: the first rocSOLVER call of a process also
+    # loads its kernels and sizes its workspace
+    t.regularize(delta)
+    t.s3ttmc_dev()
+    t.leftsvd_y()
+    t.gemmtn()
+    t.sync()
     t.timing(True)
     t.timing_reset()
     for _ in range(iters):
