@@ -65,6 +65,35 @@ def test_hoqri_order7_zipf_matches_oracle():
         assert abs(e - e64) / e64 <= 1e-6
 
 
+@pytest.mark.parametrize("alg,ofn", ALGS)
+def test_uniform_order5_reference_configuration(alg, ofn):
+    # the reference's CPU-runnable case: order 5, n = 327, 8,000 uniform
+    # tuples (deduplicated after sorting), values 1.0, rank 8, 20 iterations,
+    # delta 0.1, random start seed 0
+    inds, vals = sp.generate(sp.GEN_UNIFORM, 5, 327, 8000, 0.0, 1, seed=0)
+    assert np.all(vals == 1.0)
+    say("uniform order 5 config, alg", alg, len(vals))
+    t = sp.SpSymtensor(327, 5, inds, vals)
+    T = oracle.Tree(5, 327, inds, vals)
+    # the reference stopping rule (it stops after two iterations here)
+    r = t.tucker(alg=alg, rank=8, max_iter=20, rel_tol=1e-6, delta=0.1, seed=0)
+    Uo, eo, _ = ofn(T, 8, 20, rel_tol=1e-6, delta=0.1, seed=0)
+    assert r["iters"] == len(eo)
+    assert np.allclose(r["errors"], eo, rtol=0, atol=1e-10)
+    assert tucker.subspace_cos(r["U"], Uo) > 1 - 1e-8
+    # all 20 iterations: the fit error of this structureless tensor is
+    # 1 - 3e-8, so rounding differences between the GPU and numpy LAPACK
+    # eigen-solvers grow from iteration 12 on (to ~3e-10 by iteration 19);
+    # the first ten iterations agree to 1e-12 and every error to 1e-6
+    r = t.tucker(alg=alg, rank=8, max_iter=20, rel_tol=0.0, delta=0.1, seed=0)
+    _, eo, _ = ofn(T, 8, 20, rel_tol=0.0, delta=0.1, seed=0)
+    d = np.abs(r["errors"] - eo)
+    print("[case] uniform order 5, 20 iterations: max |error diff| per iteration", d)
+    assert len(d) == 20
+    assert d[:10].max() <= 1e-12
+    assert d.max() <= 1e-6 * eo.max()
+
+
 def test_exact_rank_recovered():
     tt, v, X, U0 = exact_rank_tensor(n=9, R=2, N=4, seed=1)
     say("tucker exact rank")
