@@ -185,23 +185,39 @@ s3ttmc_generic(GenArgs a, const T *__restrict__ U, const T *__restrict__ coef) {
   }
 }
 
-__global__ void reduce_split_rows(const SplitRow *__restrict__ split,
-                                  const double *__restrict__ partial, u64 P,
-                                  double *__restrict__ Y) {
-  // grid: (split rows, column blocks of 64); fixed summation order
+// grid: (split rows, column blocks of 64), 512 threads: wave w sums the
+// w-th eighth of the row's pieces (four interleaved chains), then the eight
+// wave sums are added in wave order -- a fixed summation order.
+constexpr int kSplitWaves = 8;
+
+__global__ void __launch_bounds__(64 * kSplitWaves)
+reduce_split_rows(const SplitRow *__restrict__ split, const double *__restrict__ partial, u64 P,
+                  double *__restrict__ Y) {
+  __shared__ double red[kSplitWaves][64];
   const SplitRow s = split[blockIdx.x];
-  const u64 J = (u64)blockIdx.y * 64 + threadIdx.x;
-  if (J >= P) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const u64 J = (u64)blockIdx.y * 64 + lane;
+  const u64 n = s.slot_end - s.slot_begin;
+  const u32 q0 = s.slot_begin + (u32)(n * w / kSplitWaves);
+  const u32 q1 = s.slot_begin + (u32)(n * (w + 1) / kSplitWaves);
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-  u32 q = s.slot_begin;
-  for (; q + 4 <= s.slot_end; q += 4) {
-    v0 += partial[(u64)q * P + J];
-    v1 += partial[(u64)(q + 1) * P + J];
-    v2 += partial[(u64)(q + 2) * P + J];
-    v3 += partial[(u64)(q + 3) * P + J];
+  if (J < P) {
+    u32 q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      v0 += partial[(u64)q * P + J];
+      v1 += partial[(u64)(q + 1) * P + J];
+      v2 += partial[(u64)(q + 2) * P + J];
+      v3 += partial[(u64)(q + 3) * P + J];
+    }
+    for (; q < q1; ++q) v0 += partial[(u64)q * P + J];
   }
-  for (; q < s.slot_end; ++q) v0 += partial[(u64)q * P + J];
-  Y[(u64)s.row * P + J] = (v0 + v1) + (v2 + v3);
+  red[w][lane] = (v0 + v1) + (v2 + v3);
+  __syncthreads();
+  if (w == 0 && J < P) {
+    double t = red[0][lane];
+    for (int i = 1; i < kSplitWaves; ++i) t += red[i][lane];
+    Y[(u64)s.row * P + J] = t;
+  }
 }
 
 template <class T>
@@ -441,8 +457,8 @@ void s3ttmc_run(Ctx &c, int precision) {
   }
   if (!pl.h_split.empty()) {
     ScopedTimer t(c, "reduce_split_rows");
-    reduce_split_rows<<<dim3((unsigned)pl.h_split.size(), (unsigned)((c.P + 63) / 64)), 64, 0,
-                        c.stream>>>(
+    reduce_split_rows<<<dim3((unsigned)pl.h_split.size(), (unsigned)((c.P + 63) / 64)),
+                        64 * kSplitWaves, 0, c.stream>>>(
         pl.d_split.as<SplitRow>(), c.d_partial.as<double>(), c.P, c.d_Y.as<double>());
     SP_CHECK_LAUNCH();
   }
