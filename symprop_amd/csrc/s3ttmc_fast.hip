@@ -243,10 +243,7 @@ __device__ __forceinline__ void load_rows(const T *U, const u32 (&tp)[M], T (&ro
 // workgroups fit a CU (160 KB) and one's producer phase overlaps the other's
 // MFMA phase.
 constexpr int fast_bo(int M, int R, int H) {
-  return 64 * ((psz(R, H) + 15) / 16 * 16 + 1 + (psz(R, M - H) + 15) / 16 * 16 + 1) * 8 >
-                 80 * 1024
-             ? 48
-             : 64;
+  return (64 * ((psz(R, H) | 1) + (psz(R, M - H) | 1)) + 16) * 8 > 80 * 1024 ? 48 : 64;
 }
 
 // TE = double: the FP64 form; TE = float: the FP32 mixed form (FP32 U rows,
@@ -267,11 +264,18 @@ s3ttmc_mfma(FastArgs a) {
   constexpr int NTILE = TH * TT;
   constexpr int TPW = (NTILE + 3) / 4;
   constexpr int BO = kernel_bo<M, R, H, TE>();
-  constexpr int LDA = TH * 16 + 1;  // odd stride: spreads the 4 row groups over banks
-  constexpr int LDB = TT * 16 + 1;
+  // Operand rows hold exactly P_H (P_T) coefficients, rounded up to an odd
+  // stride that spreads the 4 row groups over the LDS banks.  A fragment read
+  // in the padding of the last 16-wide tile returns a neighbouring owner's
+  // value; it only reaches accumulator rows h >= P_H (columns t >= P_T),
+  // which the fold never reads.  TAIL elements after the tiles keep the last
+  // owner's padding reads inside the array.
+  constexpr int LDA = PH | 1;
+  constexpr int LDB = PT | 1;
+  constexpr int TAIL = 16;
   constexpr int SA = BO * LDA, SB = BO * LDB;
   constexpr int SF = PH * PT;
-  constexpr int SMEM = (SA + SB) > SF ? (SA + SB) : SF;
+  constexpr int SMEM = (SA + SB + TAIL) > SF ? (SA + SB + TAIL) : SF;
   __shared__ TE sm[SMEM];
   TE *sA = sm;
   TE *sB = sm + SA;
@@ -333,7 +337,7 @@ s3ttmc_mfma(FastArgs a) {
       const TE one = lane < nb ? TE(1) : TE(0);
       produce_quarters<R, H, T>(
           wave, rows, cur_c, one, [&](int J, TE v) { arow[J] = v; },
-          [&](int J, TE v) { brow[J] = v; }, TH * 16, TT * 16);
+          [&](int J, TE v) { brow[J] = v; }, PH, PT);
     }
     __syncthreads();
     // ---- prefetch: U rows of batch b+1, tuple of batch b+2
