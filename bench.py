@@ -1,9 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark of the S3TTMc hot path on one MI355X.
 
-Workload (default): plain FP64 S3TTMc / HOOI on an order-4 symmetric tensor,
-n = 10,000, 1,000,000 unique sorted 4-tuples with Zipf(s=1.1) node
-popularity, values U(0,1), rank 10 (packed Y width C(12,3) = 220), seed 0.
+Workload (default): the order-6 symmetric tensor the time-to-fit target is
+stated on -- n = 50,000, 5,000,000 unique sorted 6-tuples from a planted
+8-community model (p_in = 0.8), values 1.0, rank 8 (packed Y width
+C(12,5) = 792), seed 0.  --workload zipf4 runs the order-4 Zipf(1.1) tensor
+(n = 10,000, 1M nnz, U(0,1) values, rank 10) and --workload zipf7 the
+order-7 Zipf(1.2) tensor (n = 100,000, 10M nnz, rank 6, HOQRI).
 
 A step is one S3TTMc pass (sp_s3ttmc_dev) over the tensor with the factor
 and tree resident in HBM.  W untimed warm-up steps, then K timed steps
@@ -508,7 +511,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="zipf4")
+    ap.add_argument("--workload", default="planted6")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hooi", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
