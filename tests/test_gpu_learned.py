@@ -93,7 +93,7 @@ def test_learned_start_gain_over_random_and_hoevd():
     """Order-5 planted 8-community tensors (n=1000, 500k binary tuples,
     p_in=0.9), rank 8, HOOI with delta=0.5.  The network is trained on the
     factors plain HOOI converges to on 4 training seeds (started from the
-    hypergraph sketch), then judged on 2 held-out seeds against random and
+    hypergraph sketch), then compared on 2 held-out seeds with random and
     HOEVD starts: iterations until the subspace moves by < 1e-6, converged
     fit, and angle to the planted communities."""
     N, n, nnz, K, R, delta, iters = 5, 1000, 500_000, 8, 8, 0.5, 30
