@@ -179,8 +179,11 @@ int sp_hoevd_init(sp_ctx *ctx, uint32_t R, double *U, double *eigvals);
  * matrix A (host, row-major; only its values are read): U (dim x rank,
  * row-major) with columns in ascending eigenvalue order and w (optional,
  * rank values, ascending) -- matrix::syevd, src/dense/matrixops.cpp:77-100
- * (dsyevd, then columns dim-rank..dim-1), on the context's device with
- * rocSOLVER dsyevd (dsyevj when SP_EIG=syevj).  Eigenvector signs are
+ * (dsyevd, then columns dim-rank..dim-1), on the context's device: the
+ * library's own tridiagonal reduction, bisection and inverse iteration for
+ * the `rank` wanted pairs only, and rocSOLVER dormtr (SP_EIG=syevd, syevj or
+ * tridiag-rocsolver at context creation select rocSOLVER dsyevd, dsyevj, or
+ * rocSOLVER dsytrd for the reduction).  Eigenvector signs are
  * arbitrary, as with LAPACK. */
 int sp_syevd_top(sp_ctx *ctx, const double *A, uint64_t dim, uint32_t rank, double *U,
                  double *w);

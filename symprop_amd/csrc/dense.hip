@@ -559,8 +559,55 @@ void dense_gram(Ctx &c) {
 void dense_eig_full(Ctx &c, double *G, u64 P, double *D, double *E, double *resid,
                     rocblas_int *info);
 
+// Top-R eigenpairs through the tridiagonal form: sytrd_lower (or rocSOLVER
+// sytrd) reduces G to T = Q^T G Q, the host finds T's R largest eigenpairs (bisection and
+// inverse iteration, tridiag.cpp), and rocSOLVER ormtr maps only those R
+// vectors back through Q.  The full divide-and-conquer over all P pairs and
+// the P x P back-transformation of syevd are not needed.
+void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta) {
+  rocblas_handle h = handle_for(c);
+  // workspace: D (P), E (P), tau (P), R vectors (R x P)
+  c.ws_eig.reserve((3 * P + (u64)R * P) * sizeof(double));
+  double *D = c.ws_eig.as<double>();
+  double *E = D + P;
+  double *tau = E + P;
+  double *Zd = tau + P;
+  bool lower;
+  {
+    ScopedTimer t(c, "sytrd");
+    lower = c.eig_method == 2 && sytrd_lower(c, G, P, D, E, tau);
+    if (!lower && rocsolver_dsytrd(h, rocblas_fill_upper, (rocblas_int)P, G, (rocblas_int)P,
+                                   D, E, tau) != rocblas_status_success)
+      throw Error(SP_ERR_DEVICE, "rocsolver_dsytrd failed");
+  }
+  std::vector<double> hd(2 * P), Z((u64)R * P);
+  SP_HIP(hipMemcpyAsync(hd.data(), D, 2 * P * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  theta.resize(R);
+  {
+    ScopedTimer t(c, "tridiag_top");
+    tridiag_top(hd.data(), hd.data() + P, (int)P, (int)R, theta.data(), Z.data());
+  }
+  SP_HIP(hipMemcpyAsync(Zd, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice,
+                        c.stream));
+  {
+    ScopedTimer t(c, "ormtr");
+    if (rocsolver_dormtr(h, rocblas_side_left, lower ? rocblas_fill_lower : rocblas_fill_upper,
+                         rocblas_operation_none,
+                         (rocblas_int)P, (rocblas_int)R, G, (rocblas_int)P, tau, Zd,
+                         (rocblas_int)P) != rocblas_status_success)
+      throw Error(SP_ERR_DEVICE, "rocsolver_dormtr failed");
+  }
+  transpose_nr<<<grid_for(P * R, 256), 256, 0, c.stream>>>(Zd, R, P, V);
+  SP_CHECK_LAUNCH();
+  // Z leaves host memory only after the copy has run
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
 void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta) {
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds the matrix order");
+  if (c.eig_method >= 2) return eig_top_tridiag(c, G, P, R, V, theta);
   // workspace: D (P), E (P), residual (2), then int words
   c.ws_eig.reserve((2 * P + 2) * sizeof(double) + 4 * sizeof(rocblas_int));
   double *D = c.ws_eig.as<double>();

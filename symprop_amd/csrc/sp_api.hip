@@ -83,7 +83,11 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
       c.dim = dim;
       c.nnz = nnz;
       c.norm = host_norm(order, nnz, inds, vals);
-      if (const char *e = std::getenv("SP_EIG")) c.eig_method = std::strcmp(e, "syevj") == 0;
+      if (const char *e = std::getenv("SP_EIG"))
+        c.eig_method = std::strcmp(e, "syevd") == 0               ? 0
+                       : std::strcmp(e, "syevj") == 0              ? 1
+                       : std::strcmp(e, "tridiag-rocsolver") == 0  ? 3
+                                                                   : 2;
       SP_HIP(hipGetDevice(&c.device));
       SP_HIP(hipStreamCreate(&c.stream));
       c.d_inds.alloc(nnz * order * sizeof(uint32_t));

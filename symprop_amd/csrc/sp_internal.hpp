@@ -171,10 +171,14 @@ struct Ctx {
   DevBuf ws_pr;    // P x R (top eigenvectors of the Gram)
   DevBuf ws_nr;    // n x R
   DevBuf ws_rr;    // R x R
-  DevBuf ws_eig;   // 2P + R doubles + info words
+  DevBuf ws_eig;   // eigen-step vectors (D, E, tau, R top vectors) + info words
   DevBuf ws_tiles; // Gram tile list
-  int eig_method = 0;  // 0: rocSOLVER syevd (divide and conquer), 1: syevj (Jacobi)
+  int eig_method = 2;  // 2 (default): own tridiagonal reduction (rocSOLVER sytrd
+                       // beyond its LDS limit) + host top-R tridiagonal solve +
+                       // rocSOLVER ormtr; 3: as 2 with rocSOLVER sytrd always;
+                       // 0: rocSOLVER syevd (divide and conquer); 1: syevj
   DevBuf ws_sym;   // symmetric matrix uploaded by sp_syevd_top
+  DevBuf ws_trd;   // tridiagonal reduction: broadcast vectors, barrier words
 
   // chunking of the work list for the current R: plan[0] with 64-owner
   // batches (generic, FP32 and BF16 kernels), plan[1] with the FP64 fast
@@ -241,6 +245,16 @@ void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out);
 // rocSOLVER (c.eig_method): V (device, P x R row-major) and theta (host),
 // descending.
 void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta);
+
+// Tridiagonal reduction of the symmetric P x P G (device) in one launch of
+// co-resident workgroups (sytrd.hip), LAPACK's uplo = lower layout in G, D, E, tau.  Returns
+// false, with nothing done, when P does not fit the kernel's LDS budget.
+bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau);
+
+// R largest eigenpairs of the symmetric tridiagonal T = tridiag(e, d, e)
+// (tridiag.cpp, host): w descending, Z holds R unit vectors of length P,
+// vector j at Z + j*P.
+void tridiag_top(const double *d, const double *e, int P, int R, double *w, double *Z);
 
 // learned warm start (learned.hip)
 struct Mlp;

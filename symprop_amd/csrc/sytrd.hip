@@ -1,0 +1,278 @@
+// Reduction of the symmetric P x P Gram to tridiagonal form, G = Q T Q^T,
+// in one launch of co-resident workgroups (the eigen-step's first stage; the
+// step follows matrix::syevd, src/dense/matrixops.cpp:77-100).
+//
+// Arithmetic of LAPACK dsytd2 with uplo = 'L': for column k, the reflector
+// H_k = I - tau_k v v^T (v_{k+1} = 1) of dlarfg annihilates A[k+2:, k]; then
+// p = tau_k A22 v, w = p - (tau_k/2)(p.v) v and A22 -= v w^T + w v^T.  The
+// output layout is LAPACK's (and rocSOLVER's) for uplo = lower, so that
+// ormtr applies Q: d (P), e (P-1), tau (P-1), v_k's tail in column k of G
+// below the subdiagonal (column-major).
+//
+// Layout: row i of G lives in the LDS of workgroup i mod W for the whole
+// reduction (full rows, so a row's symv and rank-2 update read only local
+// memory).  Per column there is one exchange through global memory: every
+// workgroup publishes its rows of p, its share of p.v, and (the owner of row
+// k+1) row k+1 as it stands, then counts itself in and waits for the rest.  Each
+// workgroup then applies the update to row k+1 itself and derives the next
+// reflector from it, with the same order of operations everywhere, so no
+// second exchange is needed to share v.  All reductions have a fixed order,
+// so the result is deterministic.  Each wait is bounded: on a time-out the
+// kernel raises a flag, every workgroup leaves at its next wait, and the host
+// reports an error.
+#include "sp_internal.hpp"
+
+#include <algorithm>
+
+#ifndef SP_SYTRD_ROWS
+#define SP_SYTRD_ROWS 16  // target rows of G per workgroup
+#endif
+
+namespace spamd {
+
+namespace {
+
+constexpr int kTrdThreads = 256;
+constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr int kTrdPer = 8;  // row elements per thread in the gather: P <= 2048
+constexpr size_t kTrdLds = 160 * 1024;  // LDS of a CU
+
+struct TrdArgs {
+  double *G;
+  int P, W, nloc;
+  double *d, *e, *tau;
+  double *pbuf;     // [2][P]: p of the column (by column parity)
+  double *xbuf;     // [2][P]: row k+1 before column k's update (by parity)
+  double *part;     // [2][W]: each workgroup's share of p.v (by parity)
+  unsigned *ctr;    // arrivals at the exchanges (zeroed before the launch)
+  int *abort;       // raised by a wait that timed out
+  unsigned spin_limit;
+};
+
+__device__ inline double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Sum over the workgroup in a fixed order; every thread gets the result.
+__device__ inline double block_sum(double x, double *red) {
+  x = wave_sum(x);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < kTrdWaves; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+// dlarfg on x[k+1:] (x = row k of the partly reduced matrix): v (v[k+1] = 1)
+// into LDS, tau and beta returned.  Every workgroup runs this on the same
+// row with the same order of operations, so all hold the same reflector.
+__device__ inline void reflector(const double *x, int k, int P, double *v, double *red,
+                                 double &tk, double &beta) {
+  double ss = 0.0;
+  for (int j = k + 2 + (int)threadIdx.x; j < P; j += kTrdThreads) ss += x[j] * x[j];
+  ss = block_sum(ss, red);
+  const double alpha = x[k + 1];
+  double scal = 0.0;
+  beta = alpha;
+  tk = 0.0;
+  if (ss > 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+    tk = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  for (int j = k + 1 + (int)threadIdx.x; j < P; j += kTrdThreads)
+    v[j] = j == k + 1 ? 1.0 : x[j] * scal;
+  __syncthreads();
+}
+
+// Waits until this wave's stores have been acknowledged (s_waitcnt
+// vmcnt(0); expcnt and lgkmcnt left at their maxima), i.e. have reached the
+// L2, the level the workgroup's single release below writes back.
+__device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Publishes this workgroup's data of column `epoch - 1` and waits until every
+// workgroup has done so.  One agent-scope release and one acquire per
+// workgroup (not per wave): each of these is an L2-wide writeback or
+// invalidate, and per wave they cost more than the exchange itself.
+// Returns false when the wait timed out (here or in another workgroup).
+__device__ bool exchange(const TrdArgs &a, unsigned epoch, int *fail) {
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    atomicAdd(a.ctr, 1u);
+    const unsigned target = epoch * (unsigned)a.W;
+    int f = 0;
+    for (unsigned n = 1;
+         __hip_atomic_load(a.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++n) {
+      if (n >= a.spin_limit) {
+        atomicExch(a.abort, 1);
+        f = 1;
+        break;
+      }
+      if ((n & 63) == 0 &&
+          __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        f = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *fail = f;
+  }
+  __syncthreads();
+  return *fail == 0;
+}
+
+// One exchange per column: after column k's exchange every workgroup holds
+// all of p, and row k+1 as it was before column k's update; each applies the
+// update to that row and to its own rows, and derives the next reflector
+// from the updated row itself.
+__global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
+  extern __shared__ double lds[];
+  __shared__ double red[kTrdWaves];
+  __shared__ int fail;
+  const int P = a.P, W = a.W, wg = blockIdx.x, t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  double *A = lds;                      // local row s is global row wg + s*W
+  double *v = A + (size_t)a.nloc * P;  // [P] reflector of the column
+  double *w = v + P;                    // [P]
+  double *x = w + P;                    // [P] pivot row
+  double *rc = x + P;                   // [nloc] p_i v_i of the local rows
+  const int nrows = wg < P ? (P - wg + W - 1) / W : 0;
+  for (int s = 0; s < nrows; ++s) {
+    const double *src = a.G + (size_t)(wg + s * W) * P;
+    for (int j = t; j < P; j += kTrdThreads) A[(size_t)s * P + j] = src[j];
+  }
+  for (int j = t; j < P; j += kTrdThreads) x[j] = a.G[j];
+  __syncthreads();
+  for (int k = 0; k + 1 < P; ++k) {
+    const int par = k & 1;
+    double tk, beta;
+    reflector(x, k, P, v, red, tk, beta);
+    // p = tau A v over the local rows i >= k+1, one wave per row
+    const int s0 = k + 1 <= wg ? 0 : (k + 1 - wg + W - 1) / W;
+    double *pb = a.pbuf + (size_t)par * P;
+    for (int s = s0 + wave; s < nrows; s += kTrdWaves) {
+      const double *row = A + (size_t)s * P;
+      double acc = 0.0;
+      for (int j = k + 1 + lane; j < P; j += 64) acc += row[j] * v[j];
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        const int i = wg + s * W;
+        const double p = tk * acc;
+        pb[i] = p;
+        rc[s] = p * v[i];
+      }
+    }
+    double *xb = a.xbuf + (size_t)par * P;
+    if ((k + 1) % W == wg) {
+      const double *row = A + (size_t)((k + 1) / W) * P;
+      for (int j = k + 1 + t; j < P; j += kTrdThreads) xb[j] = row[j];
+    }
+    __syncthreads();
+    if (t == 0) {
+      double c = 0.0;
+      for (int s = s0; s < nrows; ++s) c += rc[s];
+      a.part[(size_t)par * W + wg] = c;
+    }
+    if (!exchange(a, (unsigned)k + 1, &fail)) return;
+    // gather p, the pivot row and the shares of p.v: every load of this
+    // thread is issued before the first is used (one memory round trip)
+    double rp[kTrdPer], rx[kTrdPer];
+#pragma unroll
+    for (int u = 0; u < kTrdPer; ++u) {
+      const int j = k + 1 + t + u * kTrdThreads;
+      rp[u] = j < P ? pb[j] : 0.0;
+      rx[u] = j < P ? xb[j] : 0.0;
+    }
+    const double share = t < W ? a.part[(size_t)par * W + t] : 0.0;
+    if (k % W == wg) {  // column k of the output, into row k's memory of G,
+                        // which only this workgroup read (and all read row 0
+                        // before they counted in at the first exchange)
+      for (int j = k + 2 + t; j < P; j += kTrdThreads) a.G[(size_t)k * P + j] = v[j];
+      if (t == 0) {
+        a.d[k] = x[k];
+        a.e[k] = beta;
+        a.tau[k] = tk;
+      }
+    }
+    const double dot = block_sum(share, red);
+    const double al = -0.5 * tk * dot;
+#pragma unroll
+    for (int u = 0; u < kTrdPer; ++u) {
+      const int j = k + 1 + t + u * kTrdThreads;
+      if (j < P) w[j] = rp[u] + al * v[j];
+    }
+    __syncthreads();
+    {
+      const double vk = v[k + 1], wk = w[k + 1];
+#pragma unroll
+      for (int u = 0; u < kTrdPer; ++u) {
+        const int j = k + 1 + t + u * kTrdThreads;
+        if (j < P) x[j] = rx[u] - (vk * w[j] + wk * v[j]);
+      }
+    }
+    for (int s = s0; s < nrows; ++s) {
+      double *row = A + (size_t)s * P;
+      const int i = wg + s * W;
+      const double vi = v[i], wi = w[i];
+      for (int j = k + 1 + t; j < P; j += kTrdThreads) row[j] -= vi * w[j] + wi * v[j];
+    }
+    __syncthreads();
+  }
+  if ((P - 1) % W == wg && t == 0) a.d[P - 1] = x[P - 1];
+}
+
+}  // namespace
+
+bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
+  if (P < 2 || P > (u64)kTrdThreads * kTrdPer) return false;
+  int ncu = 0;
+  SP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+  // about SP_SYTRD_ROWS rows per workgroup, more workgroups when the rows do
+  // not fit the LDS; one workgroup per CU at most (co-residency) and at most
+  // one share of p.v per thread of the final sum
+  // the LDS a workgroup may take: 160 KiB less the kernel's static part
+  hipFuncAttributes fa;
+  SP_HIP(hipFuncGetAttributes(&fa, (const void *)sytrd_lower_kernel));
+  const size_t budget = kTrdLds - fa.sharedSizeBytes;
+  const int wmax = std::min(ncu, kTrdThreads);
+  int W = (int)std::min<u64>((P + SP_SYTRD_ROWS - 1) / SP_SYTRD_ROWS, (u64)wmax);
+  W = std::max(W, 1);
+  auto lds_of = [P](int w) {
+    const u64 nl = (P + w - 1) / w;
+    return (size_t)((nl * P + 3 * P + nl) * sizeof(double));
+  };
+  while (W < wmax && lds_of(W) > budget) ++W;
+  const size_t lds = lds_of(W);
+  if (lds > budget) return false;
+  const int nloc = (int)((P + W - 1) / W);
+  // scratch: pbuf (2P), xbuf (2P), part (2W), then the counter and abort words
+  c.ws_trd.reserve((4 * P + 2 * W) * sizeof(double) + 2 * sizeof(unsigned));
+  double *pbuf = c.ws_trd.as<double>();
+  double *xbuf = pbuf + 2 * P;
+  double *part = xbuf + 2 * P;
+  unsigned *ctr = reinterpret_cast<unsigned *>(part + 2 * W);
+  int *abort = reinterpret_cast<int *>(ctr + 1);
+  SP_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), c.stream));
+  SP_HIP(hipFuncSetAttribute((const void *)sytrd_lower_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // a wait polls for about a second before it gives up
+  TrdArgs a{G, (int)P, W, nloc, D, E, tau, pbuf, xbuf, part, ctr, abort, 1u << 21};
+  // An ordinary launch: W <= CUs workgroups, each within one CU's LDS, are
+  // resident together on the device.  (A cooperative launch would also
+  // reserve the hardware wave-sync unit, which one process at a time can
+  // hold; the exchange here is in software and does not need it.)
+  sytrd_lower_kernel<<<W, kTrdThreads, lds, c.stream>>>(a);
+  SP_CHECK_LAUNCH();
+  int h_abort = 0;
+  SP_HIP(hipMemcpyAsync(&h_abort, abort, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  if (h_abort) throw Error(SP_ERR_DEVICE, "tridiagonal reduction: exchange wait timed out");
+  return true;
+}
+
+}  // namespace spamd

@@ -293,12 +293,15 @@ def spectrum_matrix(P, lam, seed):
     return (Q * lam) @ Q.T
 
 
-@pytest.mark.parametrize("eig", ["syevd", "syevj"])
+@pytest.mark.parametrize("eig", ["syevd", "syevj", "tridiag", "tridiag-rocsolver"])
 @pytest.mark.parametrize("P,R,kind", [(1, 1, "random"), (2, 1, "random"), (3, 2, "random"),
                                       (50, 8, "random"), (220, 10, "gram"),
-                                      (462, 6, "clusters"), (300, 8, "rank3")])
+                                      (462, 6, "clusters"), (300, 8, "rank3"),
+                                      (792, 8, "gram"), (1716, 8, "gram")])
 def test_syevd_top_matches_numpy(P, R, kind, eig, monkeypatch):
     # matrix::syevd: top-R eigenvectors, ascending, against numpy's LAPACK
+    if eig == "syevj" and P > 500:
+        pytest.skip("Jacobi sweeps at this order take seconds; the order is covered by the others")
     monkeypatch.setenv("SP_EIG", eig)
     rng = np.random.default_rng(P + R)
     if kind == "random":
@@ -330,7 +333,7 @@ def test_syevd_top_matches_numpy(P, R, kind, eig, monkeypatch):
         assert c.min() > 1 - 1e-10
 
 
-@pytest.mark.parametrize("eig", ["syevd", "syevj"])
+@pytest.mark.parametrize("eig", ["syevd", "syevj", "tridiag", "tridiag-rocsolver"])
 def test_leftsvd_subspace(eig, monkeypatch):
     monkeypatch.setenv("SP_EIG", eig)
     rng = np.random.default_rng(22)
