@@ -14,6 +14,13 @@ import numpy as np
 import oracle
 
 
+def fit_error(norm, core_norm):
+    """sqrt(||X||^2 - ||C||^2) / ||X|| (tucker.cpp:40-45, spsymtensor.cpp:230-238)
+    with the negative rounding residue of an exact fit clamped to 0 (the
+    reference takes its sqrt, NaN)."""
+    return np.sqrt(max(norm ** 2 - core_norm ** 2, 0.0)) / norm
+
+
 def regularize(U, delta):
     U = U.copy()
     nr = np.sqrt((U * U).sum(axis=1))
@@ -51,7 +58,7 @@ def hooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
         Y = unpack_rows(T.s3ttmc(U, R, nthreads=4), R, N - 1)
         U = np.linalg.svd(Y, full_matrices=False)[0][:, :R]
         core = U.T @ Y
-        err = np.sqrt(norm ** 2 - (core ** 2).sum()) / norm
+        err = fit_error(norm, np.sqrt((core ** 2).sum()))
         errors.append(err)
         if abs(prev - err) < rel_tol:
             break
@@ -74,7 +81,7 @@ def hoqri(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
         core = U.T @ Y
         U = np.linalg.qr((Y * W) @ core.T)[0]
         cn = np.sqrt((core ** 2 * W).sum())
-        err = np.sqrt(norm ** 2 - cn ** 2) / norm
+        err = fit_error(norm, cn)
         errors.append(err)
         if abs(prev - err) < rel_tol:
             break
@@ -97,7 +104,7 @@ def althooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
         U = np.linalg.eigh(A)[1][:, n - R:]
         core = U.T @ Y
         cn = np.sqrt((core ** 2 * W).sum())
-        err = np.sqrt(norm ** 2 - cn ** 2) / norm
+        err = fit_error(norm, cn)
         errors.append(err)
         if abs(prev - err) < rel_tol:
             break

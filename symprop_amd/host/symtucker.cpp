@@ -74,6 +74,14 @@ real_t plain_norm(const std::vector<real_t> &x) {
   return std::sqrt(n);
 }
 
+// sqrt(||X||^2 - ||C||^2) / ||X|| (spsymtensor.cpp:230-238, tucker.cpp:40-45).
+// For an exactly fitted tensor the difference is a rounding residue of either
+// sign; the reference's sqrt of a negative residue is NaN, which no stopping
+// rule accepts.  It is clamped to 0 here.
+real_t fit_error(real_t norm, real_t core_norm) {
+  return std::sqrt(std::max(norm * norm - core_norm * core_norm, real_t(0))) / norm;
+}
+
 real_t auto_switch_tol(const Config &c) {
   if (c.switch_tol > 0) return c.switch_tol;
   const real_t t = (c.precision == SP_PREC_BF16) ? 1e-3 : 1e-5;
@@ -223,9 +231,7 @@ SymTuckerDecomp::SymTuckerDecomp(const Config &config, dim_t dim, order_t order)
 
 // tucker.cpp:40-45
 real_t SymTuckerDecomp::reconstruct_error(const SpSymtensor &target) const {
-  const real_t sp = target.norm();
-  const real_t cn = plain_norm(_core);
-  return std::sqrt(sp * sp - cn * cn) / sp;
+  return fit_error(target.norm(), plain_norm(_core));
 }
 
 // tucker.cpp:47-61, including its core indexing _core[i * _rank + j]
@@ -385,7 +391,7 @@ SymTuckerDecomp SpSymtensor::TuckerHOQRI(const Config &c) const {
     timed("qr", _ctx, [&] { ck(sp_qr(_ctx)); });
     const real_t rel_res = symtens_rel_res(core, core_prev, coeff, R);
     const real_t core_norm = symtens_norm(core, coeff, R);
-    const real_t error = std::sqrt(_norm * _norm - core_norm * core_norm) / _norm;
+    const real_t error = fit_error(_norm, core_norm);
     if (c.verbose)
       std::cout << std::scientific << std::setprecision(12) << "Iteration: " << iter
                 << " relative residual: " << rel_res << " error " << error << "\n";
@@ -438,7 +444,7 @@ SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
     timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
     std::copy(core_p.begin(), core_p.end(), core.begin());
     const real_t core_norm = symtens_norm(core_p, coeff, R);
-    const real_t error = std::sqrt(_norm * _norm - core_norm * core_norm) / _norm;
+    const real_t error = fit_error(_norm, core_norm);
     if (c.verbose)
       std::cout << std::scientific << std::setprecision(10) << "Iteration: " << iter
                 << " relative residual: " << rel_res << " error " << error << "\n";
