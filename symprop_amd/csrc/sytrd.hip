@@ -34,7 +34,8 @@ namespace {
 
 constexpr int kTrdThreads = 256;
 constexpr int kTrdWaves = kTrdThreads / 64;
-constexpr int kTrdPer = 8;  // row elements per thread in the gather: P <= 2048
+constexpr int kTrdMaxPer = 8;  // row elements per thread: P <= 2048
+constexpr int kTrdGroup = 4;   // rows per pass of the row loops
 constexpr size_t kTrdLds = 160 * 1024;  // LDS of a CU
 
 struct TrdArgs {
@@ -49,9 +50,33 @@ struct TrdArgs {
   unsigned spin_limit;
 };
 
+// One DPP move of a double (two 32-bit halves); C is the DPP control word.
+template <int C>
+__device__ inline double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, C, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), C, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline double read_lane(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a full wave, the same value in every lane, in a
+// fixed order: within each quad (quad_perm [1,0,3,2], then [2,3,0,1]), within
+// each row of 16 (row_ror 4, then 8), then the four rows' sums.  DPP moves
+// are VALU operations; shuffles through the LDS permute unit would cost an
+// LDS round trip per step.
 __device__ inline double wave_sum(double x) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  x += dpp_move<0xb1>(x);
+  x += dpp_move<0x4e>(x);
+  x += dpp_move<0x124>(x);
+  x += dpp_move<0x128>(x);
+  return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
 }
 
 // Sum over the workgroup in a fixed order; every thread gets the result.
@@ -126,10 +151,75 @@ __device__ bool exchange(const TrdArgs &a, unsigned epoch, int *fail) {
   return *fail == 0;
 }
 
+// p_i = tau (A v)_i for the G local rows s, s+4, ... (one wave; lanes over
+// the columns j >= k+1, v read once per column for all G rows); stores p_i
+// and returns sum p_i v_i in row order.
+template <int G>
+__device__ inline double symv_rows(const double *A, int P, int W, int wg, int s, int k,
+                                   const double *v, double tk, int lane, double *pb) {
+  const double *row[G];
+  double acc[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    row[g] = A + (size_t)(s + g * kTrdWaves) * P;
+    acc[g] = 0.0;
+  }
+  for (int j = k + 1 + lane; j < P; j += 64) {
+    const double vj = v[j];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] += row[g][j] * vj;
+  }
+  double pv = 0.0;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int i = wg + (s + g * kTrdWaves) * W;
+    const double p = tk * wave_sum(acc[g]);
+    pv += p * v[i];
+    if (lane == 0) pb[i] = p;
+  }
+  return pv;
+}
+
+// A[i][j] -= v_i w_j + w_i v_j for the G local rows s..s+G-1 and this
+// thread's columns (v_j, w_j in rv, rw); all loads of the group are issued
+// before its first store.
+template <int G, int PER>
+__device__ inline void update_rows(double *A, int P, int W, int wg, int s, int k, int t,
+                                   const double *v, const double *w, const double *rv,
+                                   const double *rw) {
+  double r[G][PER], vi[G], wi[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const double *row = A + (size_t)(s + g) * P;
+    vi[g] = v[wg + (s + g) * W];
+    wi[g] = w[wg + (s + g) * W];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = k + 1 + t + u * kTrdThreads;
+      r[g][u] = j < P ? row[j] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    double *row = A + (size_t)(s + g) * P;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = k + 1 + t + u * kTrdThreads;
+      if (j < P) row[j] = r[g][u] - (vi[g] * rw[u] + wi[g] * rv[u]);
+    }
+  }
+}
+
 // One exchange per column: after column k's exchange every workgroup holds
 // all of p, and row k+1 as it was before column k's update; each applies the
 // update to that row and to its own rows, and derives the next reflector
 // from the updated row itself.
+//
+// PER: row elements per thread (P <= 256*PER).  Each thread owns columns
+// j = k+1+t+256u, u < PER, of every row it touches outside the symv; the row
+// loops take kTrdGroup rows at a time while that many remain, so the LDS
+// round trips of a group overlap.
+template <int PER>
 __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
   extern __shared__ double lds[];
   __shared__ double red[kTrdWaves];
@@ -140,7 +230,7 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
   double *v = A + (size_t)a.nloc * P;  // [P] reflector of the column
   double *w = v + P;                    // [P]
   double *x = w + P;                    // [P] pivot row
-  double *rc = x + P;                   // [nloc] p_i v_i of the local rows
+  double *rc = x + P;                   // [waves] p_i v_i of each wave's rows
   const int nrows = wg < P ? (P - wg + W - 1) / W : 0;
   for (int s = 0; s < nrows; ++s) {
     const double *src = a.G + (size_t)(wg + s * W) * P;
@@ -152,21 +242,16 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
     const int par = k & 1;
     double tk, beta;
     reflector(x, k, P, v, red, tk, beta);
-    // p = tau A v over the local rows i >= k+1, one wave per row
+    // p = tau A v over the local rows i >= k+1: wave q takes rows s0+q,
+    // s0+q+4, ..., kTrdGroup of them per pass over v
     const int s0 = k + 1 <= wg ? 0 : (k + 1 - wg + W - 1) / W;
     double *pb = a.pbuf + (size_t)par * P;
-    for (int s = s0 + wave; s < nrows; s += kTrdWaves) {
-      const double *row = A + (size_t)s * P;
-      double acc = 0.0;
-      for (int j = k + 1 + lane; j < P; j += 64) acc += row[j] * v[j];
-      acc = wave_sum(acc);
-      if (lane == 0) {
-        const int i = wg + s * W;
-        const double p = tk * acc;
-        pb[i] = p;
-        rc[s] = p * v[i];
-      }
-    }
+    double pv = 0.0;  // this wave's rows' p_i v_i, in row order
+    int sq = s0 + wave;
+    for (; sq + (kTrdGroup - 1) * kTrdWaves < nrows; sq += kTrdGroup * kTrdWaves)
+      pv += symv_rows<kTrdGroup>(A, P, W, wg, sq, k, v, tk, lane, pb);
+    for (; sq < nrows; sq += kTrdWaves) pv += symv_rows<1>(A, P, W, wg, sq, k, v, tk, lane, pb);
+    if (lane == 0) rc[wave] = pv;
     double *xb = a.xbuf + (size_t)par * P;
     if ((k + 1) % W == wg) {
       const double *row = A + (size_t)((k + 1) / W) * P;
@@ -175,15 +260,15 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
     __syncthreads();
     if (t == 0) {
       double c = 0.0;
-      for (int s = s0; s < nrows; ++s) c += rc[s];
+      for (int q = 0; q < kTrdWaves; ++q) c += rc[q];
       a.part[(size_t)par * W + wg] = c;
     }
     if (!exchange(a, (unsigned)k + 1, &fail)) return;
     // gather p, the pivot row and the shares of p.v: every load of this
     // thread is issued before the first is used (one memory round trip)
-    double rp[kTrdPer], rx[kTrdPer];
+    double rp[PER], rx[PER];
 #pragma unroll
-    for (int u = 0; u < kTrdPer; ++u) {
+    for (int u = 0; u < PER; ++u) {
       const int j = k + 1 + t + u * kTrdThreads;
       rp[u] = j < P ? pb[j] : 0.0;
       rx[u] = j < P ? xb[j] : 0.0;
@@ -201,26 +286,27 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
     }
     const double dot = block_sum(share, red);
     const double al = -0.5 * tk * dot;
+    double rv[PER], rw[PER];
 #pragma unroll
-    for (int u = 0; u < kTrdPer; ++u) {
+    for (int u = 0; u < PER; ++u) {
       const int j = k + 1 + t + u * kTrdThreads;
-      if (j < P) w[j] = rp[u] + al * v[j];
+      rv[u] = j < P ? v[j] : 0.0;
+      rw[u] = rp[u] + al * rv[u];
+      if (j < P) w[j] = rw[u];
     }
     __syncthreads();
     {
       const double vk = v[k + 1], wk = w[k + 1];
 #pragma unroll
-      for (int u = 0; u < kTrdPer; ++u) {
+      for (int u = 0; u < PER; ++u) {
         const int j = k + 1 + t + u * kTrdThreads;
-        if (j < P) x[j] = rx[u] - (vk * w[j] + wk * v[j]);
+        if (j < P) x[j] = rx[u] - (vk * rw[u] + wk * rv[u]);
       }
     }
-    for (int s = s0; s < nrows; ++s) {
-      double *row = A + (size_t)s * P;
-      const int i = wg + s * W;
-      const double vi = v[i], wi = w[i];
-      for (int j = k + 1 + t; j < P; j += kTrdThreads) row[j] -= vi * w[j] + wi * v[j];
-    }
+    int su = s0;
+    for (; su + kTrdGroup - 1 < nrows; su += kTrdGroup)
+      update_rows<kTrdGroup, PER>(A, P, W, wg, su, k, t, v, w, rv, rw);
+    for (; su < nrows; ++su) update_rows<1, PER>(A, P, W, wg, su, k, t, v, w, rv, rw);
     __syncthreads();
   }
   if ((P - 1) % W == wg && t == 0) a.d[P - 1] = x[P - 1];
@@ -229,22 +315,27 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
 }  // namespace
 
 bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
-  if (P < 2 || P > (u64)kTrdThreads * kTrdPer) return false;
+  if (P < 2 || P > (u64)kTrdThreads * kTrdMaxPer) return false;
+  const int per = P <= 256 ? 1 : P <= 512 ? 2 : P <= 1024 ? 4 : 8;
+  const void *kern = per == 1   ? (const void *)sytrd_lower_kernel<1>
+                     : per == 2 ? (const void *)sytrd_lower_kernel<2>
+                     : per == 4 ? (const void *)sytrd_lower_kernel<4>
+                                : (const void *)sytrd_lower_kernel<8>;
   int ncu = 0;
   SP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+  // the LDS a workgroup may take: 160 KiB less the kernel's static part
+  hipFuncAttributes fa;
+  SP_HIP(hipFuncGetAttributes(&fa, kern));
+  const size_t budget = kTrdLds - fa.sharedSizeBytes;
   // about SP_SYTRD_ROWS rows per workgroup, more workgroups when the rows do
   // not fit the LDS; one workgroup per CU at most (co-residency) and at most
   // one share of p.v per thread of the final sum
-  // the LDS a workgroup may take: 160 KiB less the kernel's static part
-  hipFuncAttributes fa;
-  SP_HIP(hipFuncGetAttributes(&fa, (const void *)sytrd_lower_kernel));
-  const size_t budget = kTrdLds - fa.sharedSizeBytes;
   const int wmax = std::min(ncu, kTrdThreads);
   int W = (int)std::min<u64>((P + SP_SYTRD_ROWS - 1) / SP_SYTRD_ROWS, (u64)wmax);
   W = std::max(W, 1);
   auto lds_of = [P](int w) {
     const u64 nl = (P + w - 1) / w;
-    return (size_t)((nl * P + 3 * P + nl) * sizeof(double));
+    return (size_t)((nl * P + 3 * P + kTrdWaves) * sizeof(double));
   };
   while (W < wmax && lds_of(W) > budget) ++W;
   const size_t lds = lds_of(W);
@@ -258,15 +349,15 @@ bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
   unsigned *ctr = reinterpret_cast<unsigned *>(part + 2 * W);
   int *abort = reinterpret_cast<int *>(ctr + 1);
   SP_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), c.stream));
-  SP_HIP(hipFuncSetAttribute((const void *)sytrd_lower_kernel,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   // a wait polls for about a second before it gives up
   TrdArgs a{G, (int)P, W, nloc, D, E, tau, pbuf, xbuf, part, ctr, abort, 1u << 21};
   // An ordinary launch: W <= CUs workgroups, each within one CU's LDS, are
   // resident together on the device.  (A cooperative launch would also
   // reserve the hardware wave-sync unit, which one process at a time can
   // hold; the exchange here is in software and does not need it.)
-  sytrd_lower_kernel<<<W, kTrdThreads, lds, c.stream>>>(a);
+  void *args[] = {&a};
+  SP_HIP(hipLaunchKernel(kern, dim3(W), dim3(kTrdThreads), args, lds, c.stream));
   SP_CHECK_LAUNCH();
   int h_abort = 0;
   SP_HIP(hipMemcpyAsync(&h_abort, abort, sizeof(int), hipMemcpyDeviceToHost, c.stream));
