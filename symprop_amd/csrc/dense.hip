@@ -593,14 +593,15 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
                         c.stream));
   {
     ScopedTimer t(c, "ormtr");
-    if (rocsolver_dormtr(h, rocblas_side_left, lower ? rocblas_fill_lower : rocblas_fill_upper,
-                         rocblas_operation_none,
-                         (rocblas_int)P, (rocblas_int)R, G, (rocblas_int)P, tau, Zd,
-                         (rocblas_int)P) != rocblas_status_success)
-      throw Error(SP_ERR_DEVICE, "rocsolver_dormtr failed");
+    if (!(lower && apply_q_lower(c, G, tau, P, Zd, R, V))) {
+      if (rocsolver_dormtr(h, rocblas_side_left, lower ? rocblas_fill_lower : rocblas_fill_upper,
+                           rocblas_operation_none, (rocblas_int)P, (rocblas_int)R, G,
+                           (rocblas_int)P, tau, Zd, (rocblas_int)P) != rocblas_status_success)
+        throw Error(SP_ERR_DEVICE, "rocsolver_dormtr failed");
+      transpose_nr<<<grid_for(P * R, 256), 256, 0, c.stream>>>(Zd, R, P, V);
+      SP_CHECK_LAUNCH();
+    }
   }
-  transpose_nr<<<grid_for(P * R, 256), 256, 0, c.stream>>>(Zd, R, P, V);
-  SP_CHECK_LAUNCH();
   // Z leaves host memory only after the copy has run
   SP_HIP(hipStreamSynchronize(c.stream));
 }

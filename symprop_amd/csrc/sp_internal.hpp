@@ -251,6 +251,12 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
 // false, with nothing done, when P does not fit the kernel's LDS budget.
 bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau);
 
+// V = Q Z (V row-major P x R, Z column-major P x R, both device) for the Q of
+// sytrd_lower's output in G and tau (sytrd.hip); queued on the context's
+// stream.  Returns false, with nothing done, for P outside 2..2048.
+bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const double *Z, u32 R,
+                   double *V);
+
 // R largest eigenpairs of the symmetric tridiagonal T = tridiag(e, d, e)
 // (tridiag.cpp, host): w descending, Z holds R unit vectors of length P,
 // vector j at Z + j*P.

@@ -312,7 +312,109 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
   if ((P - 1) % W == wg && t == 0) a.d[P - 1] = x[P - 1];
 }
 
+// C = Q Z for Q = H_0 H_1 ... H_{P-2} of sytrd_lower (LAPACK dormtr, side
+// left, uplo lower, no transpose).  Each wave owns one column of Z, held in
+// registers (lane l has rows l + 64m, m < M); the reflectors are applied from
+// the last to the first, staged B at a time through LDS by the whole
+// workgroup (each expanded to a full column: zeros above row k+1, the
+// implicit 1 at k+1, the stored tail below), with 32 loads in flight per
+// thread.  Z is column-major P x R; the result goes to V, row-major P x R.
+template <int M>
+__global__ __launch_bounds__(256) void apply_q_lower_kernel(const double *__restrict__ G,
+                                                            const double *__restrict__ tau,
+                                                            int P, int B,
+                                                            const double *__restrict__ Z,
+                                                            int R, double *__restrict__ V) {
+  constexpr int PER = M / 4;        // row elements per thread when staging
+  constexpr int QU = 32 / PER;      // reflectors per staging round
+  extern __shared__ double vl[];    // [B][P] reflectors, then [B] their tau
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int r = blockIdx.x * kTrdWaves + wave;
+  const bool active = r < R;
+  double z[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int i = lane + 64 * m;
+    z[m] = active && i < P ? Z[(size_t)r * P + i] : 0.0;
+  }
+  for (int kb = P - 2; kb >= 0; kb -= B) {
+    const int nb = kb + 1 < B ? kb + 1 : B;  // reflectors kb, kb-1, ..., kb-nb+1
+    double *tl = vl + (size_t)B * P;
+    if (t < nb) tl[t] = tau[kb - t];
+    for (int q0 = 0; q0 < nb; q0 += QU) {
+      double tmp[QU][PER];
+#pragma unroll
+      for (int qq = 0; qq < QU; ++qq) {
+        const int q = q0 + qq, k = kb - q;
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+          const int i = t + kTrdThreads * c;
+          tmp[qq][c] = q < nb && i < P ? (i > k + 1 ? G[(size_t)k * P + i] : i == k + 1 ? 1.0 : 0.0)
+                                       : 0.0;
+        }
+      }
+#pragma unroll
+      for (int qq = 0; qq < QU; ++qq) {
+        const int q = q0 + qq;
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+          const int i = t + kTrdThreads * c;
+          if (q < nb && i < P) vl[(size_t)q * P + i] = tmp[qq][c];
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int q = 0; q < nb; ++q) {
+        const double *vq = vl + (size_t)q * P;
+        double vv[M], y = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int i = lane + 64 * m;
+          vv[m] = i < P ? vq[i] : 0.0;
+          y += vv[m] * z[m];
+        }
+        const double ty = tl[q] * wave_sum(y);
+#pragma unroll
+        for (int m = 0; m < M; ++m) z[m] -= ty * vv[m];
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int i = lane + 64 * m;
+      if (i < P) V[(size_t)i * R + r] = z[m];
+    }
+  }
+}
+
 }  // namespace
+
+bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const double *Z, u32 R,
+                   double *V) {
+  if (P < 2 || P > 2048 || R == 0) return false;
+  const int M = P <= 256 ? 4 : P <= 512 ? 8 : P <= 1024 ? 16 : 32;
+  const void *kern = M == 4    ? (const void *)apply_q_lower_kernel<4>
+                     : M == 8  ? (const void *)apply_q_lower_kernel<8>
+                     : M == 16 ? (const void *)apply_q_lower_kernel<16>
+                               : (const void *)apply_q_lower_kernel<32>;
+  hipFuncAttributes fa;
+  SP_HIP(hipFuncGetAttributes(&fa, kern));
+  // reflectors per LDS stage: as many as fit, at most 32 (one tau each)
+  const size_t budget = kTrdLds - fa.sharedSizeBytes;
+  const int B = (int)std::min<u64>(32, budget / ((P + 1) * sizeof(double)));
+  if (B < 1) return false;
+  const size_t lds = (size_t)B * (P + 1) * sizeof(double);
+  SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int Pi = (int)P, Ri = (int)R;
+  void *args[] = {(void *)&G, (void *)&tau, &Pi, (void *)&B, (void *)&Z, &Ri, (void *)&V};
+  SP_HIP(hipLaunchKernel(kern, dim3((R + kTrdWaves - 1) / kTrdWaves), dim3(kTrdThreads), args,
+                        lds, c.stream));
+  SP_CHECK_LAUNCH();
+  return true;
+}
 
 bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
   if (P < 2 || P > (u64)kTrdThreads * kTrdMaxPer) return false;
