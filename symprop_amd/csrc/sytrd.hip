@@ -32,10 +32,15 @@ namespace spamd {
 
 namespace {
 
-constexpr int kTrdThreads = 256;
+#ifndef SP_TRD_THREADS
+#define SP_TRD_THREADS 512  // threads per workgroup of the reduction
+#endif
+
+constexpr int kTrdThreads = SP_TRD_THREADS;
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kTrdMaxPer = 8;  // row elements per thread: P <= 2048
-constexpr int kTrdGroup = 4;   // rows per pass of the row loops
+constexpr int kTrdGroup = 4;   // rows per pass of the update loop
+constexpr int kSymvGroup = kTrdWaves >= 8 ? 2 : 4;  // rows per wave per pass of the symv
 constexpr size_t kTrdLds = 160 * 1024;  // LDS of a CU
 
 struct TrdArgs {
@@ -243,13 +248,13 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
     double tk, beta;
     reflector(x, k, P, v, red, tk, beta);
     // p = tau A v over the local rows i >= k+1: wave q takes rows s0+q,
-    // s0+q+4, ..., kTrdGroup of them per pass over v
+    // s0+q+waves, ..., kSymvGroup of them per pass over v
     const int s0 = k + 1 <= wg ? 0 : (k + 1 - wg + W - 1) / W;
     double *pb = a.pbuf + (size_t)par * P;
     double pv = 0.0;  // this wave's rows' p_i v_i, in row order
     int sq = s0 + wave;
-    for (; sq + (kTrdGroup - 1) * kTrdWaves < nrows; sq += kTrdGroup * kTrdWaves)
-      pv += symv_rows<kTrdGroup>(A, P, W, wg, sq, k, v, tk, lane, pb);
+    for (; sq + (kSymvGroup - 1) * kTrdWaves < nrows; sq += kSymvGroup * kTrdWaves)
+      pv += symv_rows<kSymvGroup>(A, P, W, wg, sq, k, v, tk, lane, pb);
     for (; sq < nrows; sq += kTrdWaves) pv += symv_rows<1>(A, P, W, wg, sq, k, v, tk, lane, pb);
     if (lane == 0) rc[wave] = pv;
     double *xb = a.xbuf + (size_t)par * P;
@@ -319,8 +324,11 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
 // workgroup (each expanded to a full column: zeros above row k+1, the
 // implicit 1 at k+1, the stored tail below), with 32 loads in flight per
 // thread.  Z is column-major P x R; the result goes to V, row-major P x R.
+constexpr int kAqThreads = 256;  // back-transform: 4 waves, one column of Z each
+constexpr int kAqWaves = kAqThreads / 64;
+
 template <int M>
-__global__ __launch_bounds__(256) void apply_q_lower_kernel(const double *__restrict__ G,
+__global__ __launch_bounds__(kAqThreads) void apply_q_lower_kernel(const double *__restrict__ G,
                                                             const double *__restrict__ tau,
                                                             int P, int B,
                                                             const double *__restrict__ Z,
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(256) void apply_q_lower_kernel(const double *__rest
   constexpr int QU = 32 / PER;      // reflectors per staging round
   extern __shared__ double vl[];    // [B][P] reflectors, then [B] their tau
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int r = blockIdx.x * kTrdWaves + wave;
+  const int r = blockIdx.x * kAqWaves + wave;
   const bool active = r < R;
   double z[M];
 #pragma unroll
@@ -348,7 +356,7 @@ __global__ __launch_bounds__(256) void apply_q_lower_kernel(const double *__rest
         const int q = q0 + qq, k = kb - q;
 #pragma unroll
         for (int c = 0; c < PER; ++c) {
-          const int i = t + kTrdThreads * c;
+          const int i = t + kAqThreads * c;
           tmp[qq][c] = q < nb && i < P ? (i > k + 1 ? G[(size_t)k * P + i] : i == k + 1 ? 1.0 : 0.0)
                                        : 0.0;
         }
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(256) void apply_q_lower_kernel(const double *__rest
         const int q = q0 + qq;
 #pragma unroll
         for (int c = 0; c < PER; ++c) {
-          const int i = t + kTrdThreads * c;
+          const int i = t + kAqThreads * c;
           if (q < nb && i < P) vl[(size_t)q * P + i] = tmp[qq][c];
         }
       }
@@ -410,15 +418,18 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
   SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int Pi = (int)P, Ri = (int)R;
   void *args[] = {(void *)&G, (void *)&tau, &Pi, (void *)&B, (void *)&Z, &Ri, (void *)&V};
-  SP_HIP(hipLaunchKernel(kern, dim3((R + kTrdWaves - 1) / kTrdWaves), dim3(kTrdThreads), args,
-                        lds, c.stream));
+  SP_HIP(hipLaunchKernel(kern, dim3((R + kAqWaves - 1) / kAqWaves), dim3(kAqThreads), args, lds,
+                        c.stream));
   SP_CHECK_LAUNCH();
   return true;
 }
 
 bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
   if (P < 2 || P > (u64)kTrdThreads * kTrdMaxPer) return false;
-  const int per = P <= 256 ? 1 : P <= 512 ? 2 : P <= 1024 ? 4 : 8;
+  const int per = P <= (u64)kTrdThreads       ? 1
+                  : P <= 2 * (u64)kTrdThreads ? 2
+                  : P <= 4 * (u64)kTrdThreads ? 4
+                                              : 8;
   const void *kern = per == 1   ? (const void *)sytrd_lower_kernel<1>
                      : per == 2 ? (const void *)sytrd_lower_kernel<2>
                      : per == 4 ? (const void *)sytrd_lower_kernel<4>
