@@ -297,7 +297,8 @@ def spectrum_matrix(P, lam, seed):
 @pytest.mark.parametrize("P,R,kind", [(1, 1, "random"), (2, 1, "random"), (3, 2, "random"),
                                       (50, 8, "random"), (220, 10, "gram"),
                                       (462, 6, "clusters"), (300, 8, "rank3"),
-                                      (792, 8, "gram"), (1716, 8, "gram")])
+                                      (792, 8, "gram"), (1716, 8, "gram"),
+                                      (2002, 8, "gram")])  # beyond the LDS-resident reduction
 def test_syevd_top_matches_numpy(P, R, kind, eig, monkeypatch):
     # matrix::syevd: top-R eigenvectors, ascending, against numpy's LAPACK
     if eig == "syevj" and P > 500:
