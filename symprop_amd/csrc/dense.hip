@@ -572,10 +572,16 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
   double *E = D + P;
   double *tau = E + P;
   double *Zd = tau + P;
+  // the reflectors: in Vr (lower form, G only read) from the own reduction,
+  // in G (upper form) from rocSOLVER's
+  c.ws_trd_v.reserve(P * P * sizeof(double));
+  double *Vr = c.ws_trd_v.as<double>();
   bool lower;
   {
     ScopedTimer t(c, "sytrd");
-    lower = c.eig_method == 2 && sytrd_lower(c, G, P, D, E, tau);
+    const u64 timeouts = c.trd_timeouts;
+    lower = c.eig_method == 2 && sytrd_lower(c, G, P, D, E, tau, Vr);
+    if (c.trd_timeouts != timeouts) ScopedTimer mark(c, "sytrd_timeout");  // counted fallback
     if (!lower && rocsolver_dsytrd(h, rocblas_fill_upper, (rocblas_int)P, G, (rocblas_int)P,
                                    D, E, tau) != rocblas_status_success)
       throw Error(SP_ERR_DEVICE, "rocsolver_dsytrd failed");
@@ -593,10 +599,11 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
                         c.stream));
   {
     ScopedTimer t(c, "ormtr");
-    if (!(lower && apply_q_lower(c, G, tau, P, Zd, R, V))) {
+    if (!(lower && apply_q_lower(c, Vr, tau, P, Zd, R, V))) {
       if (rocsolver_dormtr(h, rocblas_side_left, lower ? rocblas_fill_lower : rocblas_fill_upper,
-                           rocblas_operation_none, (rocblas_int)P, (rocblas_int)R, G,
-                           (rocblas_int)P, tau, Zd, (rocblas_int)P) != rocblas_status_success)
+                           rocblas_operation_none, (rocblas_int)P, (rocblas_int)R,
+                           lower ? Vr : G, (rocblas_int)P, tau, Zd,
+                           (rocblas_int)P) != rocblas_status_success)
         throw Error(SP_ERR_DEVICE, "rocsolver_dormtr failed");
       transpose_nr<<<grid_for(P * R, 256), 256, 0, c.stream>>>(Zd, R, P, V);
       SP_CHECK_LAUNCH();

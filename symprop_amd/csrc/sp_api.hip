@@ -1,6 +1,7 @@
 // C-ABI entry points (declared in include/symprop_amd.h).
 #include "sp_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -83,6 +84,8 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
       c.dim = dim;
       c.nnz = nnz;
       c.norm = host_norm(order, nnz, inds, vals);
+      if (const char *e = std::getenv("SP_TRD_SPIN"))
+        c.trd_spin_limit = (unsigned)std::max(1L, std::strtol(e, nullptr, 10));
       if (const char *e = std::getenv("SP_EIG"))
         c.eig_method = std::strcmp(e, "syevd") == 0               ? 0
                        : std::strcmp(e, "syevj") == 0              ? 1

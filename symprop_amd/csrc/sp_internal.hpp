@@ -179,6 +179,11 @@ struct Ctx {
                        // 0: rocSOLVER syevd (divide and conquer); 1: syevj
   DevBuf ws_sym;   // symmetric matrix uploaded by sp_syevd_top
   DevBuf ws_trd;   // tridiagonal reduction: broadcast vectors, barrier words
+  DevBuf ws_trd_v; // tridiagonal reduction: reflector tails (P x P)
+  // polls of one exchange wait before the reduction gives up (about a second;
+  // SP_TRD_SPIN at context creation sets it, to test the fallback)
+  unsigned trd_spin_limit = 1u << 21;
+  u64 trd_timeouts = 0;  // reductions that timed out and fell back
 
   // chunking of the work list for the current R: plan[0] with 64-owner
   // batches (generic, FP32 and BF16 kernels), plan[1] with the FP64 fast
@@ -246,14 +251,16 @@ void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out);
 // descending.
 void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta);
 
-// Tridiagonal reduction of the symmetric P x P G (device) in one launch of
-// co-resident workgroups (sytrd.hip), LAPACK's uplo = lower layout in G, D, E, tau.  Returns
-// false, with nothing done, when P does not fit the kernel's LDS budget.
-bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau);
+// Tridiagonal reduction of the symmetric P x P G (device, only read) in one
+// launch of co-resident workgroups (sytrd.hip): D, E, tau and the reflector
+// tails in Vr (P x P) in LAPACK's uplo = lower layout.  Returns false when P
+// does not fit the kernel's LDS budget or a wait between its workgroups
+// timed out.
+bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr);
 
 // V = Q Z (V row-major P x R, Z column-major P x R, both device) for the Q of
-// sytrd_lower's output in G and tau (sytrd.hip); queued on the context's
-// stream.  Returns false, with nothing done, for P outside 2..2048.
+// sytrd_lower's output in Vr (passed as G) and tau (sytrd.hip); queued on the
+// context's stream.  Returns false, with nothing done, for P outside 2..2048.
 bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const double *Z, u32 R,
                    double *V);
 

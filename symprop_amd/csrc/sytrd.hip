@@ -6,8 +6,8 @@
 // H_k = I - tau_k v v^T (v_{k+1} = 1) of dlarfg annihilates A[k+2:, k]; then
 // p = tau_k A22 v, w = p - (tau_k/2)(p.v) v and A22 -= v w^T + w v^T.  The
 // output layout is LAPACK's (and rocSOLVER's) for uplo = lower, so that
-// ormtr applies Q: d (P), e (P-1), tau (P-1), v_k's tail in column k of G
-// below the subdiagonal (column-major).
+// ormtr applies Q: d (P), e (P-1), tau (P-1), v_k's tail in column k of a
+// P x P array below the subdiagonal (column-major).  G itself is only read.
 //
 // Layout: row i of G lives in the LDS of workgroup i mod W for the whole
 // reduction (full rows, so a row's symv and rank-2 update read only local
@@ -19,7 +19,7 @@
 // second exchange is needed to share v.  All reductions have a fixed order,
 // so the result is deterministic.  Each wait is bounded: on a time-out the
 // kernel raises a flag, every workgroup leaves at its next wait, and the host
-// reports an error.
+// reports that the reduction was not done (G is never written).
 #include "sp_internal.hpp"
 
 #include <algorithm>
@@ -44,7 +44,8 @@ constexpr int kSymvGroup = kTrdWaves >= 8 ? 2 : 4;  // rows per wave per pass of
 constexpr size_t kTrdLds = 160 * 1024;  // LDS of a CU
 
 struct TrdArgs {
-  double *G;
+  const double *G;  // the P x P matrix, read only
+  double *Vr;       // reflector tails: v_k's rows k+2.. in column k (column-major)
   int P, W, nloc;
   double *d, *e, *tau;
   double *pbuf;     // [2][P]: p of the column (by column parity)
@@ -279,10 +280,8 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
       rx[u] = j < P ? xb[j] : 0.0;
     }
     const double share = t < W ? a.part[(size_t)par * W + t] : 0.0;
-    if (k % W == wg) {  // column k of the output, into row k's memory of G,
-                        // which only this workgroup read (and all read row 0
-                        // before they counted in at the first exchange)
-      for (int j = k + 2 + t; j < P; j += kTrdThreads) a.G[(size_t)k * P + j] = v[j];
+    if (k % W == wg) {  // column k of the output
+      for (int j = k + 2 + t; j < P; j += kTrdThreads) a.Vr[(size_t)k * P + j] = v[j];
       if (t == 0) {
         a.d[k] = x[k];
         a.e[k] = beta;
@@ -424,7 +423,7 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
   return true;
 }
 
-bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
+bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr) {
   if (P < 2 || P > (u64)kTrdThreads * kTrdMaxPer) return false;
   const int per = P <= (u64)kTrdThreads       ? 1
                   : P <= 2 * (u64)kTrdThreads ? 2
@@ -463,8 +462,7 @@ bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
   int *abort = reinterpret_cast<int *>(ctr + 1);
   SP_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), c.stream));
   SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  // a wait polls for about a second before it gives up
-  TrdArgs a{G, (int)P, W, nloc, D, E, tau, pbuf, xbuf, part, ctr, abort, 1u << 21};
+  TrdArgs a{G, Vr, (int)P, W, nloc, D, E, tau, pbuf, xbuf, part, ctr, abort, c.trd_spin_limit};
   // An ordinary launch: W <= CUs workgroups, each within one CU's LDS, are
   // resident together on the device.  (A cooperative launch would also
   // reserve the hardware wave-sync unit, which one process at a time can
@@ -475,7 +473,12 @@ bool sytrd_lower(Ctx &c, double *G, u64 P, double *D, double *E, double *tau) {
   int h_abort = 0;
   SP_HIP(hipMemcpyAsync(&h_abort, abort, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
-  if (h_abort) throw Error(SP_ERR_DEVICE, "tridiagonal reduction: exchange wait timed out");
+  if (h_abort) {
+    // some workgroup was not resident in time (the device is shared with other
+    // work): every workgroup has left and G is untouched; the caller falls back
+    ++c.trd_timeouts;
+    return false;
+  }
   return true;
 }
 

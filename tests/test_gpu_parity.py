@@ -334,6 +334,32 @@ def test_syevd_top_matches_numpy(P, R, kind, eig, monkeypatch):
         assert c.min() > 1 - 1e-10
 
 
+def test_syevd_top_reduction_timeout_falls_back(monkeypatch):
+    """A tiny spin limit on the reduction's own bounded waits (set from the
+    host) makes its workgroups give up; the library restores the matrix,
+    counts the time-out and reduces it with rocSOLVER instead, with the same
+    results."""
+    monkeypatch.setenv("SP_TRD_SPIN", "1")
+    P, R = 792, 8
+    rng = np.random.default_rng(5)
+    lam = np.sort(np.concatenate([1e5 * 0.5 ** np.arange(R + 4), rng.random(P - R - 4)]))
+    A = spectrum_matrix(P, lam, 7)
+    say("syevd_top timeout fallback", P, R)
+    t = sp.SpSymtensor(4, 3, np.array([[1, 2, 3]], dtype=np.uint32), np.ones(1))
+    t.timing(True)
+    fallbacks = 0
+    for _ in range(3):
+        U, w = t.syevd_top(A, R)
+        lam_ref, V = np.linalg.eigh(A)
+        assert np.allclose(w, lam_ref[P - R:], rtol=0, atol=1e-12 * lam.max())
+        assert np.abs(A @ U - U * w).max() <= 1e-11 * lam.max()
+        c = np.linalg.svd(U.T @ V[:, P - R:], compute_uv=False)
+        assert c.min() > 1 - 1e-10
+    _, fallbacks = t.timing_get("sytrd_timeout")
+    t.timing(False)
+    assert fallbacks >= 1
+
+
 @pytest.mark.parametrize("eig", ["syevd", "syevj", "tridiag", "tridiag-rocsolver"])
 def test_leftsvd_subspace(eig, monkeypatch):
     monkeypatch.setenv("SP_EIG", eig)
