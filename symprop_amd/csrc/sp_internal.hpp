@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/symprop_amd.h"
+#include "tridiag.hpp"
 
 namespace spamd {
 
@@ -263,11 +264,6 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
 // context's stream.  Returns false, with nothing done, for P outside 2..2048.
 bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const double *Z, u32 R,
                    double *V);
-
-// R largest eigenpairs of the symmetric tridiagonal T = tridiag(e, d, e)
-// (tridiag.cpp, host): w descending, Z holds R unit vectors of length P,
-// vector j at Z + j*P.
-void tridiag_top(const double *d, const double *e, int P, int R, double *w, double *Z);
 
 // learned warm start (learned.hip)
 struct Mlp;

@@ -1,15 +1,16 @@
 // Host part of the eigen-step: the R largest eigenpairs of a symmetric
 // tridiagonal matrix T (diagonal d[0..P), off-diagonal e[0..P-1)), the
-// reduced form rocSOLVER's sytrd leaves of the P x P Gram.  The same
+// reduced form of the P x P Gram (sytrd.hip, or rocSOLVER's sytrd).  The same
 // arithmetic as LAPACK's dstebz (Sturm-sequence bisection) and dstein
 // (inverse iteration with partial pivoting, reorthogonalisation inside
 // clusters of close eigenvalues), for the few wanted pairs only.  Every loop
 // is bounded.
-#include "sp_internal.hpp"
+#include "tridiag.hpp"
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <stdexcept>
 #include <vector>
 
 namespace spamd {
@@ -53,7 +54,7 @@ double unit_random(unsigned long long &s) {
 }  // namespace
 
 void tridiag_top(const double *d, const double *e, int P, int R, double *w, double *Z) {
-  if (P <= 0 || R <= 0 || R > P) throw Error(SP_ERR_ARG, "tridiag_top: size out of range");
+  if (P <= 0 || R <= 0 || R > P) throw std::invalid_argument("tridiag_top: size out of range");
   std::vector<double> e2(P, 0.0);
   double lo = d[0], hi = d[0], tn = 0.0, emax2 = 0.0;
   for (int i = 0; i < P; ++i) {
