@@ -13,8 +13,8 @@
 // reduction (full rows, so a row's symv and rank-2 update read only local
 // memory).  Per column there is one exchange through global memory: every
 // workgroup publishes its rows of p, its share of p.v, and (the owner of row
-// k+1) row k+1 as it stands, then counts itself in and waits for the rest.  Each
-// workgroup then applies the update to row k+1 itself and derives the next
+// k+1) row k+1 as it stands, then counts itself in and waits for the rest.
+// Each workgroup then applies the update to row k+1 itself and derives the next
 // reflector from it, with the same order of operations everywhere, so no
 // second exchange is needed to share v.  All reductions have a fixed order,
 // so the result is deterministic.  Each wait is bounded: on a time-out the
@@ -24,17 +24,18 @@
 
 #include <algorithm>
 
+// build-time tuning: rows of G per workgroup (target) and threads per
+// workgroup of the reduction
 #ifndef SP_SYTRD_ROWS
-#define SP_SYTRD_ROWS 16  // target rows of G per workgroup
+#define SP_SYTRD_ROWS 16
+#endif
+#ifndef SP_TRD_THREADS
+#define SP_TRD_THREADS 512
 #endif
 
 namespace spamd {
 
 namespace {
-
-#ifndef SP_TRD_THREADS
-#define SP_TRD_THREADS 512  // threads per workgroup of the reduction
-#endif
 
 constexpr int kTrdThreads = SP_TRD_THREADS;
 constexpr int kTrdWaves = kTrdThreads / 64;
@@ -157,7 +158,7 @@ __device__ bool exchange(const TrdArgs &a, unsigned epoch, int *fail) {
   return *fail == 0;
 }
 
-// p_i = tau (A v)_i for the G local rows s, s+4, ... (one wave; lanes over
+// p_i = tau (A v)_i for the G local rows s, s+waves, ... (one wave; lanes over
 // the columns j >= k+1, v read once per column for all G rows); stores p_i
 // and returns sum p_i v_i in row order.
 template <int G>
@@ -221,10 +222,10 @@ __device__ inline void update_rows(double *A, int P, int W, int wg, int s, int k
 // update to that row and to its own rows, and derives the next reflector
 // from the updated row itself.
 //
-// PER: row elements per thread (P <= 256*PER).  Each thread owns columns
-// j = k+1+t+256u, u < PER, of every row it touches outside the symv; the row
-// loops take kTrdGroup rows at a time while that many remain, so the LDS
-// round trips of a group overlap.
+// PER: row elements per thread (P <= kTrdThreads*PER).  Each thread owns
+// columns j = k+1+t+kTrdThreads*u, u < PER, of every row it touches outside
+// the symv; the row loops take several rows at a time while that many
+// remain, so the LDS round trips of a group overlap.
 template <int PER>
 __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
   extern __shared__ double lds[];
