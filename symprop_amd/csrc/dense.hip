@@ -573,14 +573,17 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
   double *tau = E + P;
   double *Zd = tau + P;
   // the reflectors: in Vr (lower form, G only read) from the own reduction,
-  // in G (upper form) from rocSOLVER's
-  c.ws_trd_v.reserve(P * P * sizeof(double));
-  double *Vr = c.ws_trd_v.as<double>();
+  // in G (upper form) from rocSOLVER's; Vr only where the own one can run
+  double *Vr = nullptr;
+  if (c.eig_method == 2 && P <= kTrdMaxP) {
+    c.ws_trd_v.reserve(P * P * sizeof(double));
+    Vr = c.ws_trd_v.as<double>();
+  }
   bool lower;
   {
     ScopedTimer t(c, "sytrd");
     const u64 timeouts = c.trd_timeouts;
-    lower = c.eig_method == 2 && sytrd_lower(c, G, P, D, E, tau, Vr);
+    lower = Vr && sytrd_lower(c, G, P, D, E, tau, Vr);
     if (c.trd_timeouts != timeouts) ScopedTimer mark(c, "sytrd_timeout");  // counted fallback
     if (!lower && rocsolver_dsytrd(h, rocblas_fill_upper, (rocblas_int)P, G, (rocblas_int)P,
                                    D, E, tau) != rocblas_status_success)

@@ -162,6 +162,26 @@ void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out) {
   if (n > 0x7fffffffull / n) throw Error(SP_ERR_UNSUPPORTED, "dim too large for dense HOEVD");
   DevBuf A(n * n * sizeof(double));
   hoevd_gram(c, A.as<double>());
+  if (c.eig_method >= 2) {
+    // only the R wanted pairs (eig_top: tridiagonal reduction, host bisection
+    // and inverse iteration for R pairs, back-transform of R vectors); the
+    // columns come out descending, matrix::syevd's last R columns ascending
+    DevBuf V(n * R * sizeof(double));
+    std::vector<double> theta;
+    {
+      ScopedTimer t(c, "hoevd_eig");
+      eig_top(c, A.as<double>(), n, R, V.as<double>(), theta);
+    }
+    std::vector<double> Vh(n * R);
+    SP_HIP(hipMemcpyAsync(Vh.data(), V.p, n * R * sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));
+    for (u64 i = 0; i < n; ++i)
+      for (u32 j = 0; j < R; ++j) U_out[i * R + j] = Vh[i * R + (R - 1 - j)];
+    if (eig_out)
+      for (u32 j = 0; j < R; ++j) eig_out[j] = theta[R - 1 - j];
+    return;
+  }
   // full divide-and-conquer spectrum, as matrix::syevd (LAPACKE_dsyevd)
   DevBuf W(2 * n * sizeof(double) + sizeof(rocblas_int));
   double *D = W.as<double>();

@@ -258,6 +258,7 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
 // does not fit the kernel's LDS budget or a wait between its workgroups
 // timed out.
 bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr);
+constexpr u64 kTrdMaxP = 2048;  // sytrd_lower returns false beyond this order
 
 // V = Q Z (V row-major P x R, Z column-major P x R, both device) for the Q of
 // sytrd_lower's output in Vr (passed as G) and tau (sytrd.hip); queued on the

@@ -425,7 +425,7 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
 }
 
 bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr) {
-  if (P < 2 || P > (u64)kTrdThreads * kTrdMaxPer) return false;
+  if (P < 2 || P > kTrdMaxP || P > (u64)kTrdThreads * kTrdMaxPer) return false;
   const int per = P <= (u64)kTrdThreads       ? 1
                   : P <= 2 * (u64)kTrdThreads ? 2
                   : P <= 4 * (u64)kTrdThreads ? 4

@@ -481,13 +481,18 @@ def test_hoevd_gram_hubs_and_column_tiles():
     assert np.array_equal(t.hoevd_gram(), A)
 
 
-def test_hoevd_factor_matches_oracle_eigenspace():
-    inds, vals = sp.generate(sp.GEN_ZIPF, 4, 280, 6000, 1.1, 1, seed=9)
+@pytest.mark.parametrize("eig", ["tridiag", "syevd"])
+@pytest.mark.parametrize("n,cnt", [(280, 6000), (2300, 30000)])
+def test_hoevd_factor_matches_oracle_eigenspace(n, cnt, eig, monkeypatch):
+    # n = 2300 is past the LDS-resident reduction: rocSOLVER dsytrd, then the
+    # host top-R stage and dormtr; "syevd" is the full rocSOLVER dsyevd path
+    monkeypatch.setenv("SP_EIG", eig)
+    inds, vals = sp.generate(sp.GEN_ZIPF, 4, n, cnt, 1.1, 1, seed=9)
     R = 6
-    say("hoevd factor", len(vals))
-    t = sp.SpSymtensor(280, 4, inds, vals)
+    say("hoevd factor", n, len(vals), eig)
+    t = sp.SpSymtensor(n, 4, inds, vals)
     U, w = t.hoevd_init(R)
-    Uo, wo = oracle.hoevd_factor(oracle.hoevd_gram(4, 280, inds, vals), R)
+    Uo, wo = oracle.hoevd_factor(oracle.hoevd_gram(4, n, inds, vals), R)
     assert np.allclose(w, wo, rtol=1e-12)
     assert np.all(np.diff(w) >= 0)
     assert np.allclose(U.T @ U, np.eye(R), atol=1e-12)
