@@ -15,6 +15,7 @@
 // Reductions over the n rows go through fixed-order partial sums, so results
 // are deterministic.
 #include "sp_internal.hpp"
+#include "wave.hpp"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -89,6 +90,42 @@ __global__ void sum_partials(const double *__restrict__ partial, u64 nb, u64 len
   double s = 0.0;
   for (u64 b = 0; b < nb; ++b) s += partial[b * len + t];
   out[t] = s;
+}
+
+// Largest |a_i| of each block's grid-stride share (max is exact in any order).
+__global__ void __launch_bounds__(256) absmax_partial(const double *__restrict__ a, u64 n,
+                                                      double *__restrict__ part) {
+  __shared__ double m[256];
+  double v = 0.0;
+  for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < n; i += (u64)gridDim.x * 256)
+    v = fmax(v, fabs(a[i]));
+  m[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
+__global__ void scale_by(double *__restrict__ a, u64 n, double s) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] *= s;
+}
+
+// The same sums for short outputs (a few hundred entries, where one thread
+// per entry would add its nb partials one after another): one wavefront per
+// entry, lanes over the partials, then a fixed-order wave sum.
+__global__ void __launch_bounds__(256) sum_partials_wave(const double *__restrict__ partial,
+                                                         u64 nb, u64 len,
+                                                         double *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const u64 lane = threadIdx.x & 63;
+  if (t >= len) return;  // whole wavefronts leave together
+  double s = 0.0;
+  for (u64 b = lane; b < nb; b += 64) s += partial[b * len + t];
+  s = wave_sum(s);
+  if (lane == 0) out[t] = s;
 }
 
 // out[i][r] = sum_p Y[i][p] * M[r][p]  (Y: n x P, M: R x P, out: n x R).
@@ -373,8 +410,12 @@ void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *
     atb_partial<<<dim3((unsigned)nb, (unsigned)ctiles), 256, 0, c.stream>>>(
         A, R, B, P, n, rows_per, c.d_scratch.as<double>());
     SP_CHECK_LAUNCH();
-    sum_partials<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
-        c.d_scratch.as<double>(), nb, (u64)R * P, out);
+    if ((u64)R * P <= 1024)
+      sum_partials_wave<<<(unsigned)(((u64)R * P + 3) / 4), 256, 0, c.stream>>>(
+          c.d_scratch.as<double>(), nb, (u64)R * P, out);
+    else
+      sum_partials<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
+          c.d_scratch.as<double>(), nb, (u64)R * P, out);
     SP_CHECK_LAUNCH();
   }
 }
@@ -616,9 +657,35 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
   SP_HIP(hipStreamSynchronize(c.stream));
 }
 
+int scale_to_unit(Ctx &c, double *A, u64 count) {
+  constexpr unsigned kBlocks = 256;
+  c.ws_amax.reserve(kBlocks * sizeof(double));
+  absmax_partial<<<kBlocks, 256, 0, c.stream>>>(A, count, c.ws_amax.as<double>());
+  SP_CHECK_LAUNCH();
+  double part[kBlocks];
+  SP_HIP(hipMemcpyAsync(part, c.ws_amax.p, sizeof(part), hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  double amax = 0.0;
+  for (double q : part) amax = std::max(amax, q);
+  if (!(amax > 0.0 && std::isfinite(amax) && (amax < 0x1p-400 || amax > 0x1p400))) return 0;
+  const int ex = -std::ilogb(amax);
+  scale_by<<<grid_for(count, 256), 256, 0, c.stream>>>(A, count, std::ldexp(1.0, ex));
+  SP_CHECK_LAUNCH();
+  return ex;
+}
+
 void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta) {
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds the matrix order");
-  if (c.eig_method >= 2) return eig_top_tridiag(c, G, P, R, V, theta);
+  // A Gram far outside [2^-400, 2^400] (e.g. from a nearly vanishing Y) is
+  // scaled by a power of two first (exact), as LAPACK's dsyevd scales its
+  // input (rocSOLVER's does not), so that Householder sums of squares and
+  // pivots stay in range; the eigenvalues are scaled back.
+  const int ex = scale_to_unit(c, G, P * P);
+  if (c.eig_method >= 2) {
+    eig_top_tridiag(c, G, P, R, V, theta);
+    for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(theta[r], -ex);
+    return;
+  }
   // workspace: D (P), E (P), residual (2), then int words
   c.ws_eig.reserve((2 * P + 2) * sizeof(double) + 4 * sizeof(rocblas_int));
   double *D = c.ws_eig.as<double>();
@@ -630,7 +697,7 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
   SP_HIP(hipMemcpyAsync(lam.data(), D, P * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
   theta.resize(R);
-  for (u32 r = 0; r < R; ++r) theta[r] = lam[P - 1 - r];
+  for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(lam[P - 1 - r], -ex);
   top_columns<<<grid_for(P * R, 256), 256, 0, c.stream>>>(G, P, R, V);
   SP_CHECK_LAUNCH();
 }

@@ -25,6 +25,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <cmath>
 #include <vector>
 
 namespace spamd {
@@ -182,7 +183,9 @@ void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out) {
       for (u32 j = 0; j < R; ++j) eig_out[j] = theta[R - 1 - j];
     return;
   }
-  // full divide-and-conquer spectrum, as matrix::syevd (LAPACKE_dsyevd)
+  // full divide-and-conquer spectrum, as matrix::syevd (LAPACKE_dsyevd), on
+  // A scaled into range by a power of two (LAPACK's dsyevd scales its input)
+  const int ex = scale_to_unit(c, A.as<double>(), n * n);
   DevBuf W(2 * n * sizeof(double) + sizeof(rocblas_int));
   double *D = W.as<double>();
   double *E = D + n;
@@ -210,6 +213,8 @@ void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out) {
     SP_HIP(hipMemcpyAsync(eig_out, D + (n - R), R * sizeof(double), hipMemcpyDeviceToHost,
                           c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
+  if (eig_out)
+    for (u32 j = 0; j < R; ++j) eig_out[j] = std::ldexp(eig_out[j], -ex);
 }
 
 }  // namespace spamd

@@ -181,6 +181,7 @@ struct Ctx {
   DevBuf ws_sym;   // symmetric matrix uploaded by sp_syevd_top
   DevBuf ws_trd;   // tridiagonal reduction: broadcast vectors, barrier words
   DevBuf ws_trd_v; // tridiagonal reduction: reflector tails (P x P)
+  DevBuf ws_amax;  // eigen-step: per-block maxima of |G|
   // polls of one exchange wait before the reduction gives up (about a second;
   // SP_TRD_SPIN at context creation sets it, to test the fallback)
   unsigned trd_spin_limit = 1u << 21;
@@ -246,6 +247,11 @@ double dense_core_norm(Ctx &c);
 // HOEVD initial factor (hoevd.hip)
 void hoevd_gram(Ctx &c, double *A);
 void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out);
+
+// Scales the device array A (count values) by 2^ex so that its largest |a|
+// lies in [1, 2) when that largest value is outside [2^-400, 2^400] (finite,
+// nonzero); returns ex (0: not scaled).
+int scale_to_unit(Ctx &c, double *A, u64 count);
 
 // Top-R eigenpairs of the P x P symmetric matrix G (device, destroyed) by
 // rocSOLVER (c.eig_method): V (device, P x R row-major) and theta (host),

@@ -21,6 +21,7 @@
 // kernel raises a flag, every workgroup leaves at its next wait, and the host
 // reports that the reduction was not done (G is never written).
 #include "sp_internal.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 
@@ -56,35 +57,6 @@ struct TrdArgs {
   int *abort;       // raised by a wait that timed out
   unsigned spin_limit;
 };
-
-// One DPP move of a double (two 32-bit halves); C is the DPP control word.
-template <int C>
-__device__ inline double dpp_move(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, C, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), C, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-__device__ inline double read_lane(double x, int lane) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// Sum over the 64 lanes of a full wave, the same value in every lane, in a
-// fixed order: within each quad (quad_perm [1,0,3,2], then [2,3,0,1]), within
-// each row of 16 (row_ror 4, then 8), then the four rows' sums.  DPP moves
-// are VALU operations; shuffles through the LDS permute unit would cost an
-// LDS round trip per step.
-__device__ inline double wave_sum(double x) {
-  x += dpp_move<0xb1>(x);
-  x += dpp_move<0x4e>(x);
-  x += dpp_move<0x124>(x);
-  x += dpp_move<0x128>(x);
-  return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
-}
 
 // Sum over the workgroup in a fixed order; every thread gets the result.
 __device__ inline double block_sum(double x, double *red) {

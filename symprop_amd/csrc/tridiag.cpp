@@ -55,6 +55,25 @@ double unit_random(unsigned long long &s) {
 
 void tridiag_top(const double *d, const double *e, int P, int R, double *w, double *Z) {
   if (P <= 0 || R <= 0 || R > P) throw std::invalid_argument("tridiag_top: size out of range");
+  // Far outside [2^-400, 2^400] the pivot floor 10 eps ||T|| and its inverse
+  // leave the normal range; solve T scaled by a power of two (exact), as
+  // LAPACK's drivers scale such matrices, and scale the eigenvalues back.
+  {
+    double tn = 0.0;
+    for (int i = 0; i < P; ++i) {
+      const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < P ? std::fabs(e[i]) : 0.0);
+      tn = std::max(tn, std::fabs(d[i]) + r);
+    }
+    if (tn > 0.0 && std::isfinite(tn) && (tn < 0x1p-400 || tn > 0x1p400)) {
+      const int ex = -std::ilogb(tn);
+      std::vector<double> ds(P), es(P > 1 ? P - 1 : 0);
+      for (int i = 0; i < P; ++i) ds[i] = std::ldexp(d[i], ex);
+      for (int i = 0; i + 1 < P; ++i) es[i] = std::ldexp(e[i], ex);
+      tridiag_top(ds.data(), es.data(), P, R, w, Z);
+      for (int j = 0; j < R; ++j) w[j] = std::ldexp(w[j], -ex);
+      return;
+    }
+  }
   std::vector<double> e2(P, 0.0);
   double lo = d[0], hi = d[0], tn = 0.0, emax2 = 0.0;
   for (int i = 0; i < P; ++i) {

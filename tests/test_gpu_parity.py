@@ -382,6 +382,48 @@ def test_leftsvd_subspace(eig, monkeypatch):
     assert np.allclose(c, 1.0, atol=1e-9)
 
 
+@pytest.mark.parametrize("eig", ["tridiag", "tridiag-rocsolver", "syevd"])
+@pytest.mark.parametrize("factor", [1e-200, 1e200])
+def test_syevd_top_far_from_unit_scale(factor, eig, monkeypatch):
+    # eigenpairs of c*A are c times A's with the same vectors; with c far from
+    # 1 the eigen-step scales the matrix by a power of two first (as LAPACK's
+    # dsyevd does), so its pivots and sums of squares stay in range
+    monkeypatch.setenv("SP_EIG", eig)
+    P, R = 220, 10
+    rng = np.random.default_rng(31)
+    lam = np.sort(np.concatenate([0.5 ** np.arange(R + 4), 1e-3 * rng.random(P - R - 4)]))
+    A = spectrum_matrix(P, lam, 3) * factor
+    say("syevd_top scaled", factor, eig)
+    t = sp.SpSymtensor(4, 3, np.array([[1, 2, 3]], dtype=np.uint32), np.ones(1))
+    U, w = t.syevd_top(A, R)
+    assert np.all(np.isfinite(U)) and np.all(np.isfinite(w))
+    assert np.allclose(w / factor, lam[P - R:], rtol=0, atol=1e-12)
+    assert np.allclose(U.T @ U, np.eye(R), atol=1e-11)
+    assert np.abs((A @ U - U * w) / factor).max() <= 1e-11
+
+
+def test_leftsvd_scale_invariant():
+    # U and 1e-40*U span the same space: for order 4 the second Y is 1e-120
+    # times the first and its Gram about 1e-240 times; leftsvd must return
+    # the same subspace, with singular values scaled by 1e-120
+    rng = np.random.default_rng(23)
+    N, n, R = 4, 80, 4
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(4000, N)), axis=1), axis=0)
+    t = sp.SpSymtensor(n, N, tt, rng.random(len(tt)))
+    U = np.linalg.qr(rng.standard_normal((n, R)))[0]
+    say("leftsvd scale invariance")
+    t.s3ttmc(U)
+    sig1 = t.leftsvd_y()
+    U1 = t.factor()
+    t.s3ttmc(1e-40 * U)
+    sig2 = t.leftsvd_y()
+    U2 = t.factor()
+    assert np.all(np.isfinite(U2))
+    assert np.allclose(sig2 / 1e-120, sig1, rtol=1e-10)
+    c = np.linalg.svd(U1.T @ U2, compute_uv=False)
+    assert np.allclose(c, 1.0, atol=1e-9)
+
+
 def test_leftsvd_rank_deficient_completes_basis():
     # one nonzero (1,2,3): Y has 3 nonzero rows, so rank(Y) <= 3 < R = 4;
     # LAPACK's SVD returns an orthonormal completion for the missing direction

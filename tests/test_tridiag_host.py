@@ -64,6 +64,9 @@ def cases():
     yield "zero", np.zeros(20), np.zeros(19), 4
     yield "identity", np.ones(20), np.zeros(19), 4
     yield "graded", 10.0 ** -np.arange(30), 10.0 ** -np.arange(1, 30), 5
+    # far below / above the normal range of the pivot floor: solved scaled
+    yield "tiny", 1e-250 * rng.standard_normal(40), 1e-250 * rng.standard_normal(39), 5
+    yield "huge", 1e250 * rng.standard_normal(40), 1e250 * rng.standard_normal(39), 5
 
 
 @pytest.mark.parametrize("name,d,e,R", list(cases()), ids=[c[0] for c in cases()])
@@ -76,7 +79,7 @@ def test_tridiag_top_matches_eigh(harness, name, d, e, R):
     # eigenvalues: the R largest, descending, to a few ulps of ||T||
     assert np.abs(w - lam[::-1][:R]).max() <= 1e-14 * scale
     # eigenpairs: residual and orthonormality
-    assert np.linalg.norm(T @ Z.T - Z.T * w, axis=0).max() <= 1e-13 * scale
+    assert np.linalg.norm((T @ Z.T - Z.T * w) / scale, axis=0).max() <= 1e-13
     assert np.abs(Z @ Z.T - np.eye(R)).max() <= 1e-12
     # the wanted subspace, where a gap separates it from the rest
     if R < P and (lam[::-1][R - 1] - lam[::-1][R]) > 1e-8 * scale:
