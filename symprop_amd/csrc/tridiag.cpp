@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 namespace spamd {
@@ -100,8 +101,8 @@ void tridiag_top(const double *d, const double *e, int P, int R, double *w, doub
   }
 
   // eigenvalues: w[j] = (j+1)-th largest, by bisection to full precision;
-  // up to 8 unconverged intervals are halved per Sturm pass
-  constexpr int K = 8;
+  // up to 8 (16 when more are open) unconverged intervals are halved per
+  // Sturm pass
   std::vector<double> a(R, lo), b(R, hi);
   std::vector<int> open(R);
   for (int it = 0; it < 256; ++it) {
@@ -112,7 +113,8 @@ void tridiag_top(const double *d, const double *e, int P, int R, double *w, doub
       if (b[j] - a[j] > tol && x > a[j] && x < b[j]) open[m++] = j;
     }
     if (m == 0) break;
-    for (int g = 0; g < m; g += K) {
+    auto halve = [&](auto kc, int g) {
+      constexpr int K = decltype(kc)::value;
       double x[K];
       int cnt[K];
       for (int k = 0; k < K; ++k) {
@@ -124,6 +126,15 @@ void tridiag_top(const double *d, const double *e, int P, int R, double *w, doub
         const int j = open[g + k];
         if (cnt[k] > P - 1 - j) b[j] = x[k];  // the wanted one is below x
         else a[j] = x[k];
+      }
+    };
+    for (int g = 0; g < m;) {
+      if (m - g > 8) {
+        halve(std::integral_constant<int, 16>(), g);
+        g += 16;
+      } else {
+        halve(std::integral_constant<int, 8>(), g);
+        g += 8;
       }
     }
   }
