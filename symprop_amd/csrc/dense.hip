@@ -92,6 +92,23 @@ __global__ void sum_partials(const double *__restrict__ partial, u64 nb, u64 len
   out[t] = s;
 }
 
+// The same sums for mid-size outputs: a block takes 64 consecutive entries
+// with 4 threads each (thread q adds partials q, q+4, ..., loads coalesced
+// across the entries), then the 4 sums of an entry in a fixed order.
+__global__ void __launch_bounds__(256) sum_partials_cols(const double *__restrict__ partial,
+                                                         u64 nb, u64 len,
+                                                         double *__restrict__ out) {
+  __shared__ double acc[4][64];
+  const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const u64 t = (u64)blockIdx.x * 64 + e;
+  double s = 0.0;
+  if (t < len)
+    for (u64 b = q; b < nb; b += 4) s += partial[b * len + t];
+  acc[q][e] = s;
+  __syncthreads();
+  if (q == 0 && t < len) out[t] = (acc[0][e] + acc[1][e]) + (acc[2][e] + acc[3][e]);
+}
+
 // Largest |a_i| of each block's grid-stride share (max is exact in any order).
 __global__ void __launch_bounds__(256) absmax_partial(const double *__restrict__ a, u64 n,
                                                       double *__restrict__ part) {
@@ -412,6 +429,9 @@ void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *
     SP_CHECK_LAUNCH();
     if ((u64)R * P <= 1024)
       sum_partials_wave<<<(unsigned)(((u64)R * P + 3) / 4), 256, 0, c.stream>>>(
+          c.d_scratch.as<double>(), nb, (u64)R * P, out);
+    else if ((u64)R * P <= 65536)
+      sum_partials_cols<<<(unsigned)(((u64)R * P + 63) / 64), 256, 0, c.stream>>>(
           c.d_scratch.as<double>(), nb, (u64)R * P, out);
     else
       sum_partials<<<grid_for((u64)R * P, 256), 256, 0, c.stream>>>(
