@@ -12,7 +12,8 @@ Properties used:
 - full symmetry of the core U^T Y unpacked to an order-N tensor;
 - additivity over a split of the nonzeros: Y(X1 + X2) = Y(X1) + Y(X2);
 - the weighted Gram identity of the packed Y;
-- determinism: repeated calls are identical.
+- determinism: repeated calls are identical;
+and the eigen-step of the HOOI update on that Gram against numpy's eigh.
 """
 import itertools
 import sys
@@ -136,3 +137,17 @@ def test_full_size_properties(which, request):
     G = t.gram()
     Gref = (Y * np.sqrt(W)).T @ (Y * np.sqrt(W))
     assert np.allclose(G, Gref, rtol=1e-10, atol=1e-12 * np.abs(Gref).max())
+    say(which, "eigen-step on the full Gram")
+    # the HOOI factor update: top-R eigenpairs of this Gram (the library's
+    # reduction, host bisection/inverse iteration, back-transform) against
+    # numpy's LAPACK eigh of the same matrix
+    lam, V = np.linalg.eigh(Gref)
+    top = lam[::-1][:R]
+    sig = t.leftsvd_y()
+    assert np.allclose(sig ** 2, top, rtol=0, atol=1e-11 * lam.max())
+    Un = t.factor()
+    assert np.allclose(Un.T @ Un, np.eye(R), atol=1e-10)
+    if top[-1] - lam[::-1][R] > 1e-6 * lam.max():
+        Z = (Y * np.sqrt(W)) @ V[:, ::-1][:, :R] / np.sqrt(top)
+        c = np.linalg.svd(Un.T @ Z, compute_uv=False)
+        assert np.allclose(c, 1.0, atol=1e-8)
