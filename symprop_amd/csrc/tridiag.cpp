@@ -100,11 +100,36 @@ void tridiag_top(const double *d, const double *e, int P, int R, double *w, doub
     return;
   }
 
-  // eigenvalues: w[j] = (j+1)-th largest, by bisection to full precision;
-  // up to 8 (16 when more are open) unconverged intervals are halved per
-  // Sturm pass
+  // eigenvalues: w[j] = (j+1)-th largest, by bisection to full precision.
+  // Up to 16 open intervals: one Sturm pass of 8 or 16 shifts spread over
+  // the distinct open intervals (several eigenvalues share one until the
+  // shifts separate them), and every shift's count narrows every open
+  // interval (multisection).  More than 16: groups of 16 midpoints.
   std::vector<double> a(R, lo), b(R, hi);
   std::vector<int> open(R);
+  auto narrow = [&](int j, double x, int cnt) {
+    if (!(x > a[j] && x < b[j])) return;
+    if (cnt > P - 1 - j) b[j] = x;  // the wanted one is below x
+    else a[j] = x;
+  };
+  auto multisect = [&](auto kc, int m) {
+    constexpr int K = decltype(kc)::value;
+    int rep[K], nd = 0;  // one open index per distinct interval, in order
+    for (int q = 0; q < m; ++q) {
+      const int j = open[q];
+      if (nd == 0 || a[j] != a[rep[nd - 1]] || b[j] != b[rep[nd - 1]]) rep[nd++] = j;
+    }
+    double x[K];
+    int cnt[K], ns = 0;
+    for (int i = 0; i < nd; ++i) {
+      const int sl = K / nd + (i < K % nd ? 1 : 0);
+      const double lo_i = a[rep[i]], w_i = b[rep[i]] - a[rep[i]];
+      for (int q = 1; q <= sl; ++q) x[ns++] = lo_i + w_i * ((double)q / (sl + 1));
+    }
+    sturm_counts<K>(d, e2.data(), P, x, pivmin, cnt);
+    for (int q = 0; q < m; ++q)
+      for (int k = 0; k < K; ++k) narrow(open[q], x[k], cnt[k]);
+  };
   for (int it = 0; it < 256; ++it) {
     int m = 0;
     for (int j = 0; j < R; ++j) {
@@ -113,28 +138,21 @@ void tridiag_top(const double *d, const double *e, int P, int R, double *w, doub
       if (b[j] - a[j] > tol && x > a[j] && x < b[j]) open[m++] = j;
     }
     if (m == 0) break;
-    auto halve = [&](auto kc, int g) {
-      constexpr int K = decltype(kc)::value;
-      double x[K];
-      int cnt[K];
-      for (int k = 0; k < K; ++k) {
-        const int j = open[std::min(g + k, m - 1)];
-        x[k] = 0.5 * (a[j] + b[j]);
-      }
-      sturm_counts<K>(d, e2.data(), P, x, pivmin, cnt);
-      for (int k = 0; k < K && g + k < m; ++k) {
-        const int j = open[g + k];
-        if (cnt[k] > P - 1 - j) b[j] = x[k];  // the wanted one is below x
-        else a[j] = x[k];
-      }
-    };
-    for (int g = 0; g < m;) {
-      if (m - g > 8) {
-        halve(std::integral_constant<int, 16>(), g);
-        g += 16;
-      } else {
-        halve(std::integral_constant<int, 8>(), g);
-        g += 8;
+    if (m <= 8) {
+      multisect(std::integral_constant<int, 8>(), m);
+    } else if (m <= 16) {
+      multisect(std::integral_constant<int, 16>(), m);
+    } else {
+      for (int g = 0; g < m; g += 16) {
+        constexpr int K = 16;
+        double x[K];
+        int cnt[K];
+        for (int k = 0; k < K; ++k) {
+          const int j = open[std::min(g + k, m - 1)];
+          x[k] = 0.5 * (a[j] + b[j]);
+        }
+        sturm_counts<K>(d, e2.data(), P, x, pivmin, cnt);
+        for (int k = 0; k < K && g + k < m; ++k) narrow(open[g + k], x[k], cnt[k]);
       }
     }
   }
