@@ -563,6 +563,7 @@ void dense_gemmtn(Ctx &c) {
   c.d_core.reserve((u64)c.R * c.P * sizeof(double));
   atb(c, c.d_U.as<double>(), c.R, c.d_Y.as<double>(), c.P, c.dim,
       c.d_core.as<double>(), "gemmtn");
+  c.core_R = c.R;
 }
 
 void dense_symgemm(Ctx &c) {

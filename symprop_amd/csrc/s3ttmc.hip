@@ -462,6 +462,7 @@ void s3ttmc_run(Ctx &c, int precision) {
         pl.d_split.as<SplitRow>(), c.d_partial.as<double>(), c.P, c.d_Y.as<double>());
     SP_CHECK_LAUNCH();
   }
+  c.y_R = R;
 }
 
 }  // namespace spamd

@@ -43,7 +43,15 @@ void need_factor(Ctx &c) {
 }
 
 void need_y(Ctx &c) {
-  need(c.d_Y.p && c.P > 0, SP_ERR_STATE, "no S3TTMc result on the device");
+  need(c.d_Y.p && c.P > 0 && c.y_R != 0, SP_ERR_STATE, "no S3TTMc result on the device");
+  need(c.y_R == c.R, SP_ERR_STATE,
+       "the device S3TTMc result was computed for another rank (run S3TTMc again)");
+}
+
+void need_core(Ctx &c) {
+  need(c.d_core.p && c.core_R != 0, SP_ERR_STATE, "no core (call sp_gemmtn first)");
+  need(c.core_R == c.R, SP_ERR_STATE,
+       "the device core was computed for another rank (call sp_gemmtn again)");
 }
 
 }  // namespace
@@ -280,7 +288,7 @@ int sp_symgemm(sp_ctx *ctx) {
     Ctx &c = ctx->c;
     need_factor(c);
     need_y(c);
-    need(c.d_core.p, SP_ERR_STATE, "no core (call sp_gemmtn first)");
+    need_core(c);
     dense_symgemm(c);
   });
 }
@@ -329,7 +337,7 @@ int sp_regularize(sp_ctx *ctx, double delta) {
 int sp_core_norm(sp_ctx *ctx, double *out) {
   return guard(ctx, [&] {
     need(ctx && out, SP_ERR_ARG, "null argument");
-    need(ctx->c.d_core.p, SP_ERR_STATE, "no core");
+    need_core(ctx->c);
     *out = dense_core_norm(ctx->c);
   });
 }

@@ -165,6 +165,10 @@ struct Ctx {
   DevBuf d_core;    // f64 R x P
   DevBuf d_W;       // f64 P  (symtens_coeff of order N-1)
   DevBuf d_G;       // f64 P x P
+  // rank the device Y and core were computed for (0: none).  A factor of
+  // another rank makes them stale: their widths no longer match P.
+  u32 y_R = 0;
+  u32 core_R = 0;
   DevBuf d_scratch;
   DevBuf d_partial;
   // persistent workspaces of the dense Y-side ops (grown, never shrunk)

@@ -13,7 +13,7 @@
 #include <mutex>
 #include <random>
 #include <stdexcept>
-#include <unordered_map>
+#include <map>
 
 namespace symprop {
 
@@ -169,14 +169,16 @@ void unpack_symtens(const real_t *packed, dim_t dim, order_t order, real_t *full
 
 // ------------------------------------------------------------ FunctionTimer
 namespace {
-struct TimeAccumulator {
-  std::mutex m;
-  std::unordered_map<std::string, long long> total, calls;
-  static TimeAccumulator &get() {
-    static TimeAccumulator a;
-    return a;
-  }
+// Per-name accumulated wall time (microseconds) and call count.
+struct TimerTotals {
+  long long micros = 0;
+  long long calls = 0;
 };
+std::mutex g_timer_mutex;
+std::map<std::string, TimerTotals> &timer_table() {
+  static std::map<std::string, TimerTotals> table;
+  return table;
+}
 }  // namespace
 
 FunctionTimer::FunctionTimer(std::string_view name)
@@ -186,36 +188,42 @@ FunctionTimer::~FunctionTimer() { stop(); }
 void FunctionTimer::stop() {
   if (m_stopped) return;
   m_stopped = true;
-  const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
-                      std::chrono::high_resolution_clock::now() - m_start)
-                      .count();
-  auto &a = TimeAccumulator::get();
-  std::lock_guard<std::mutex> lk(a.m);
-  a.total[m_name] += us;
-  a.calls[m_name]++;
+  const long long us = std::chrono::duration_cast<std::chrono::microseconds>(
+                           std::chrono::high_resolution_clock::now() - m_start)
+                           .count();
+  std::lock_guard<std::mutex> lk(g_timer_mutex);
+  TimerTotals &t = timer_table()[m_name];
+  t.micros += us;
+  t.calls += 1;
 }
 
-// function_timer.cpp:25-46 report format
+// Report layout of the reference (src/utils/function_timer.cpp:25-46): a
+// header line, then one line per name, longest total first, as
+// "%10.3f ms  | %10lld calls | name" with the total in milliseconds.
 void FunctionTimer::printTimes() {
-  auto &a = TimeAccumulator::get();
-  std::lock_guard<std::mutex> lk(a.m);
-  std::vector<std::pair<std::string, long long>> v(a.total.begin(), a.total.end());
-  std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return y.second < x.second; });
-  std::cout << "Accumulated function times:\n";
-  for (const auto &e : v) {
-    std::streamsize ss = std::cout.precision();
-    const double ms = static_cast<double>(e.second) / 1000.0;
-    std::cout << std::fixed << std::setprecision(3) << std::setw(10) << ms << " ms  | "
-              << std::setw(10) << a.calls[e.first] << " calls | " << e.first << "\n";
-    std::cout.precision(ss);
+  std::vector<std::pair<std::string, TimerTotals>> rows;
+  {
+    std::lock_guard<std::mutex> lk(g_timer_mutex);
+    rows.assign(timer_table().begin(), timer_table().end());
   }
+  std::stable_sort(rows.begin(), rows.end(), [](const auto &x, const auto &y) {
+    return x.second.micros > y.second.micros;
+  });
+  std::string text = "Accumulated function times:\n";
+  char line[64];
+  for (const auto &[name, t] : rows) {
+    std::snprintf(line, sizeof line, "%10.3f ms  | %10lld calls | ", t.micros / 1000.0,
+                  t.calls);
+    text += line;
+    text += name;
+    text += '\n';
+  }
+  std::cout << text;
 }
 
 void FunctionTimer::clear() {
-  auto &a = TimeAccumulator::get();
-  std::lock_guard<std::mutex> lk(a.m);
-  a.total.clear();
-  a.calls.clear();
+  std::lock_guard<std::mutex> lk(g_timer_mutex);
+  timer_table().clear();
 }
 
 // ------------------------------------------------------------ SymTuckerDecomp

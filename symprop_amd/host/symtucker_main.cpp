@@ -1,126 +1,172 @@
-// symtucker CLI: same flags, messages and output file format as the
-// reference driver (examples/symtucker.cpp:15-139), running on the MI355X
-// path.  Extension flag: -p <0|1|2> selects FP64 / FP32 / BF16 S3TTMc
-// iterations (reduced forms finish with FP64 iterations).
+// symtucker command-line driver on the MI355X S3TTMc path.
+//
+// Interface kept from the reference driver (examples/symtucker.cpp:29-139):
+// the flags -f -i -r -t -d -a -s -o -init -h, the usage line, the
+// "Option X is not defined" message, exit status 1 for -h and for unknown
+// flags, the "Insertion time" and per-algorithm "Running"/"Time" lines, the
+// output sinks ("" -> discarded, "stdout", "stderr", else a file path), and
+// -a -1 (default) running HOOI, HOQRI and AltHOOI in turn.  Extension flag:
+// -p <0|1|2> selects FP64 / FP32 / BF16 S3TTMc iterations (the reduced forms
+// finish with FP64 iterations).
+//
+// Design: options are rows of a table (flag, value parser), algorithms are
+// rows of a second table (id, printed name, member function), so adding a
+// flag or an algorithm is one line.
 #include "symtucker.hpp"
 
 #include "../../include/symprop_amd.h"
 
+#include <array>
 #include <chrono>
+#include <cstdio>
 #include <fstream>
+#include <functional>
 #include <iostream>
-#include <optional>
+#include <stdexcept>
 #include <string>
+#include <string_view>
 #include <vector>
 
-using namespace symprop;
+namespace {
 
-template <typename Alg>
-SymTuckerDecomp run_algorithm(Alg alg, const Config &con, const std::string &name) {
-  std::cout << "Running " << name << " algorithm" << std::endl;
-  auto t1 = std::chrono::high_resolution_clock::now();
-  auto tucker = alg(con);
-  auto t2 = std::chrono::high_resolution_clock::now();
-  auto duration = std::chrono::duration_cast<std::chrono::milliseconds>(t2 - t1).count();
-  std::cout << "---------- Time: " << duration << " ms ----------" << std::endl;
-  FunctionTimer::printTimes();
-  FunctionTimer::clear();
-  return tucker;
-}
+using symprop::Config;
+using symprop::FunctionTimer;
+using symprop::SpSymtensor;
+using symprop::SymTuckerDecomp;
 
-void help(const char *prog) {
-  std::cout << prog
-            << " -f <input_file> -i <max_iter> -r <rank> -t <rel_tol> -d "
-               "<delta> -a <alg> -s <seed> -o <output_file> -init <init method>"
-            << std::endl;
-}
-
-struct Input {
+struct Invocation {
   Config config;
-  std::string filePath;
-  std::string outputFile;
-  int alg = -1;
+  std::string tensor_path;
+  std::string sink;  // where the decomposition is written
+  int alg = -1;      // -1: all algorithms
 };
 
-std::optional<Input> parse(int argc, char **argv) {
-  Input input;
-  for (int i = 1; i < argc; i++) {
-    std::string arg = argv[i];
-    if (arg == "-f" && i + 1 < argc) {
-      input.filePath = argv[++i];
-    } else if (arg == "-i" && i + 1 < argc) {
-      input.config.max_iter = (index_t)std::stod(argv[++i]);
-    } else if (arg == "-r" && i + 1 < argc) {
-      input.config.rank = std::stoi(argv[++i]);
-    } else if (arg == "-t" && i + 1 < argc) {
-      input.config.rel_tol = std::stod(argv[++i]);
-    } else if (arg == "-d" && i + 1 < argc) {
-      input.config.delta = std::stod(argv[++i]);
-    } else if (arg == "-a" && i + 1 < argc) {
-      input.alg = std::stoi(argv[++i]);
-    } else if (arg == "-s" && i + 1 < argc) {
-      input.config.seed = std::stoi(argv[++i]);
-    } else if (arg == "-o" && i + 1 < argc) {
-      input.outputFile = argv[++i];
-    } else if (arg == "-init" && i + 1 < argc) {
-      input.config.init = argv[++i];
-    } else if (arg == "-p" && i + 1 < argc) {
-      input.config.precision = std::stoi(argv[++i]);
-    } else if (arg == "-h") {
-      help(argv[0]);
-      return std::nullopt;
-    } else {
-      std::cout << "Option " << argv[i] << " is not defined\n";
-      help(argv[0]);
-      return std::nullopt;
-    }
-  }
-  return input;
+using Setter = void (*)(Invocation &, const char *);
+
+struct OptionRow {
+  std::string_view flag;
+  Setter set;
+};
+
+// Numeric conversions follow the reference: -i goes through a double, the
+// integer flags through int.
+const std::array<OptionRow, 10> kOptions{{
+    {"-f", [](Invocation &v, const char *s) { v.tensor_path = s; }},
+    {"-i", [](Invocation &v, const char *s) {
+       v.config.max_iter = static_cast<symprop::index_t>(std::stod(s));
+     }},
+    {"-r", [](Invocation &v, const char *s) {
+       v.config.rank = static_cast<symprop::dim_t>(std::stoi(s));
+     }},
+    {"-t", [](Invocation &v, const char *s) { v.config.rel_tol = std::stod(s); }},
+    {"-d", [](Invocation &v, const char *s) { v.config.delta = std::stod(s); }},
+    {"-a", [](Invocation &v, const char *s) { v.alg = std::stoi(s); }},
+    {"-s", [](Invocation &v, const char *s) {
+       v.config.seed = static_cast<symprop::index_t>(std::stoi(s));
+     }},
+    {"-o", [](Invocation &v, const char *s) { v.sink = s; }},
+    {"-init", [](Invocation &v, const char *s) { v.config.init = s; }},
+    {"-p", [](Invocation &v, const char *s) { v.config.precision = std::stoi(s); }},
+}};
+
+struct AlgorithmRow {
+  int id;
+  const char *name;
+  SymTuckerDecomp (SpSymtensor::*run)(const Config &) const;
+};
+
+const std::array<AlgorithmRow, 3> kAlgorithms{{
+    {0, "HOOI", &SpSymtensor::TuckerHOOI},
+    {1, "HOQRI", &SpSymtensor::TuckerHOQRI},
+    {2, "AltHOOI", &SpSymtensor::TuckerAltHOOI},
+}};
+
+void print_usage(const char *prog) {
+  std::cout << prog << " -f <input_file> -i <max_iter> -r <rank> -t <rel_tol> -d <delta>"
+            << " -a <alg> -s <seed> -o <output_file> -init <init method>" << std::endl;
 }
 
-int main(int argc, char **argv) {
-  auto input = parse(argc, argv);
-  if (!input.has_value()) return 1;
+// Returns false when the driver must stop with status 1 (-h or a flag that
+// is unknown or lacks its value).
+bool read_arguments(int argc, char **argv, Invocation &inv) {
+  int i = 1;
+  while (i < argc) {
+    const std::string_view word = argv[i];
+    if (word == "-h") {
+      print_usage(argv[0]);
+      return false;
+    }
+    const OptionRow *row = nullptr;
+    for (const auto &r : kOptions)
+      if (r.flag == word) row = &r;
+    if (row == nullptr || i + 1 >= argc) {
+      std::cout << "Option " << argv[i] << " is not defined\n";
+      print_usage(argv[0]);
+      return false;
+    }
+    row->set(inv, argv[i + 1]);
+    i += 2;
+  }
+  return true;
+}
 
-  uint32_t order = 0, dim = 0;
+std::string sink_path(const std::string &sink) {
+  if (sink.empty()) return "/dev/null";
+  if (sink == "stdout" || sink == "stderr") return "/dev/" + sink;
+  return sink;
+}
+
+long long elapsed_ms(std::chrono::high_resolution_clock::time_point since) {
+  return std::chrono::duration_cast<std::chrono::milliseconds>(
+             std::chrono::high_resolution_clock::now() - since)
+      .count();
+}
+
+// Timed run of one algorithm, then the accumulated function times.
+SymTuckerDecomp run_one(const SpSymtensor &t, const AlgorithmRow &a, const Config &c) {
+  std::cout << "Running " << a.name << " algorithm" << std::endl;
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  SymTuckerDecomp result = (t.*a.run)(c);
+  std::cout << "---------- Time: " << elapsed_ms(t0) << " ms ----------" << std::endl;
+  FunctionTimer::printTimes();
+  FunctionTimer::clear();
+  return result;
+}
+
+// The tensor file through the library's loader (two passes: shape, then data).
+void load_tensor(const std::string &path, symprop::order_t &order, symprop::dim_t &dim,
+                 std::vector<symprop::dim_t> &inds, std::vector<symprop::real_t> &vals) {
   uint64_t nnz = 0;
-  if (sp_load_tns(input->filePath.c_str(), &order, &dim, &nnz, nullptr, nullptr) != SP_OK) {
-    std::cerr << "Error opening file: " << input->filePath << std::endl;
+  if (sp_load_tns(path.c_str(), &order, &dim, &nnz, nullptr, nullptr) != SP_OK) {
+    std::cerr << "Error opening file: " << path << std::endl;
     throw std::runtime_error("Error opening file in SymtensLoader");
   }
-  std::vector<dim_t> inds((size_t)nnz * order);
-  std::vector<real_t> vals(nnz);
-  sp_load_tns(input->filePath.c_str(), &order, &dim, &nnz, inds.data(), vals.data());
+  inds.resize(static_cast<size_t>(nnz) * order);
+  vals.resize(nnz);
+  if (sp_load_tns(path.c_str(), &order, &dim, &nnz, inds.data(), vals.data()) != SP_OK)
+    throw std::runtime_error(std::string("symtucker: ") + sp_last_error());
+}
 
-  auto t1 = std::chrono::high_resolution_clock::now();
-  SpSymtensor symtens(dim, order, inds, vals);
-  auto t2 = std::chrono::high_resolution_clock::now();
-  auto duration = std::chrono::duration_cast<std::chrono::milliseconds>(t2 - t1).count();
-  std::cout << "Insertion time: " << duration << " ms" << std::endl;
+}  // namespace
 
+int main(int argc, char **argv) {
+  Invocation inv;
+  if (!read_arguments(argc, argv, inv)) return 1;
+
+  symprop::order_t order = 0;
+  symprop::dim_t dim = 0;
+  std::vector<symprop::dim_t> inds;
+  std::vector<symprop::real_t> vals;
+  load_tensor(inv.tensor_path, order, dim, inds, vals);
+
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  const SpSymtensor tensor(dim, order, inds, vals);
+  std::cout << "Insertion time: " << elapsed_ms(t0) << " ms" << std::endl;
   FunctionTimer::printTimes();
   FunctionTimer::clear();
 
-  std::string outpath = input->outputFile;
-  if (outpath.empty()) outpath = "/dev/null";
-  else if (outpath == "stdout") outpath = "/dev/stdout";
-  else if (outpath == "stderr") outpath = "/dev/stderr";
-  std::ofstream out(outpath);
-
-  auto hooi = [&symtens](const Config &c) { return symtens.TuckerHOOI(c); };
-  auto hoqri = [&symtens](const Config &c) { return symtens.TuckerHOQRI(c); };
-  auto alt = [&symtens](const Config &c) { return symtens.TuckerAltHOOI(c); };
-  if (input->alg == -1) {
-    run_algorithm(hooi, input->config, "HOOI").print(out);
-    run_algorithm(hoqri, input->config, "HOQRI").print(out);
-    run_algorithm(alt, input->config, "AltHOOI").print(out);
-    return 0;
-  } else if (input->alg == 0) {
-    run_algorithm(hooi, input->config, "HOOI").print(out);
-  } else if (input->alg == 1) {
-    run_algorithm(hoqri, input->config, "HOQRI").print(out);
-  } else if (input->alg == 2) {
-    run_algorithm(alt, input->config, "AltHOOI").print(out);
-  }
+  std::ofstream out(sink_path(inv.sink));
+  for (const auto &a : kAlgorithms)
+    if (inv.alg == -1 || inv.alg == a.id) run_one(tensor, a, inv.config).print(out);
   return 0;
 }

@@ -269,6 +269,38 @@ def test_dense_ops_vs_numpy():
     assert np.allclose(np.abs(Q.T @ Qref), np.eye(R), atol=1e-8)
 
 
+def test_rank_change_makes_y_and_core_stale():
+    # A factor of another rank changes the packed width P; the device Y and
+    # core of the previous rank must then be refused, not read with the new
+    # width (they are P_old wide).
+    rng = np.random.default_rng(22)
+    N, n = 4, 40
+    tt = np.unique(np.sort(rng.integers(1, n + 1, size=(1500, N)), axis=1), axis=0)
+    v = rng.random(len(tt))
+    t = sp.SpSymtensor(n, N, tt, v)
+    say("rank change: S3TTMc at R=4, factor at R=8")
+    t.s3ttmc(np.linalg.qr(rng.standard_normal((n, 4)))[0])
+    t.gemmtn()
+    U8 = np.linalg.qr(rng.standard_normal((n, 8)))[0]
+    t.set_factor(U8)
+    for call in (t.y, t.gram, t.gemmtn, t.symgemm, t.leftsvd_y, t.core_norm):
+        with pytest.raises(sp.SpError) as e:
+            call()
+        assert e.value.code == sp.SP_ERR_STATE, call
+    # after an S3TTMc at the new rank everything works again
+    Y = t.s3ttmc(U8)
+    assert Y.shape == (n, oracle.symtens_coeff(8, N - 1).size)
+    core = t.gemmtn()
+    assert np.allclose(core, U8.T @ Y, rtol=1e-12, atol=1e-12 * np.abs(core).max())
+    t.core_norm()
+    # and back to the smaller rank: stale again until S3TTMc runs
+    t.set_factor(U8[:, :4].copy())
+    with pytest.raises(sp.SpError):
+        t.gram()
+    with pytest.raises(sp.SpError):
+        t.core_norm()
+
+
 @pytest.mark.parametrize("cond", [1.0, 1e5, 1e10])
 def test_qr_well_and_ill_conditioned(cond):
     # CholeskyQR2 for well-conditioned factors, Householder (rocSOLVER) past the
