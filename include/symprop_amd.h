@@ -98,6 +98,22 @@ int sp_factor_download(sp_ctx *ctx, double *U);
  * (include/detail/compressed_dtree.hpp:21-22).  precision: SP_PREC_*. */
 int sp_s3ttmc_dev(sp_ctx *ctx, int precision);
 
+/* Allocation-free form of sp_s3ttmc_dev.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * the same kernels on the same work plan as sp_s3ttmc_dev (bit-identical
+ * result) for the caller's device factor dU (dim x R FP64, row-major) into
+ * the caller's device dY (dim x P FP64, row-major, P = C(R+N-2, N-1)),
+ * queued on the context's stream (sp_sync waits for it); the context's own
+ * factor, Y and core are left alone.  The context must already be prepared
+ * for rank R by an sp_s3ttmc_dev call at that rank (SP_ERR_STATE otherwise).
+ * Scratch (split-row partial sums, the FP32 copy of U for the reduced
+ * precisions) lives in a static 64 MiB device array per GPU that every call
+ * on that GPU shares: calls that overlap on one GPU race, and a need beyond
+ * 64 MiB is rejected with SP_ERR_UNSUPPORTED.  Calls no hipMalloc/hipFree. */
+int sp_s3ttmc_noalloc(sp_ctx *ctx, const double *dU, uint32_t R, double *dY,
+                      int precision);
+
 /* Host-buffer convenience form of the same call: uploads U, runs, and copies
  * the packed Y into Y (n rows, leading dimension ldy >= P). */
 int sp_s3ttmc(sp_ctx *ctx, const double *U, uint32_t R, double *Y,

@@ -54,6 +54,7 @@ SIGNATURES = {
     "sp_factor_upload": (ctypes.c_int, [vp, f64p, ctypes.c_uint32]),
     "sp_factor_download": (ctypes.c_int, [vp, f64p]),
     "sp_s3ttmc_dev": (ctypes.c_int, [vp, ctypes.c_int]),
+    "sp_s3ttmc_noalloc": (ctypes.c_int, [vp, vp, ctypes.c_uint32, vp, ctypes.c_int]),
     "sp_s3ttmc": (ctypes.c_int, [vp, f64p, ctypes.c_uint32, f64p, ctypes.c_uint64,
                                  ctypes.c_int]),
     "sp_y_download": (ctypes.c_int, [vp, f64p, ctypes.c_uint64]),

@@ -367,19 +367,16 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
   c.chunk_R = R;
 }
 
-void s3ttmc_run(Ctx &c, int precision) {
-  const u32 R = c.R;
-  s3ttmc_prepare(c, R);
-  const int m = (int)c.order - 1;
-  const u64 nU = (u64)c.dim * R;
-  c.d_Y.reserve((u64)c.dim * c.P * sizeof(double));
+namespace {
 
-  GenArgs a{};
-  a.tup = c.w_tuple.as<u32>();
-  a.segptr = c.w_segptr.as<u64>();
-  a.tab_ptr = c.d_tab_ptr.as<u32>();
-  a.tab_term = c.d_tab_term.as<u32>();
-  a.fact = c.d_fact.as<double>();
+// The work plan a launch of the given precision uses.
+const Plan &plan_for(const Ctx &c, int precision) {
+  return (precision == SP_PREC_FP64 && c.fast_kernel && c.fast_bo < 64) ? c.plan[1] : c.plan[0];
+}
+
+// Packed widths per chain level and the LDS layout of the generic kernel.
+u64 generic_layout(const Ctx &c, GenArgs &a) {
+  const int m = (int)c.order - 1;
   u64 off = 0;
   for (int k = 1; k <= m; ++k) {
     a.ptr_off[k] = c.tab_ptr_off[k];
@@ -393,75 +390,102 @@ void s3ttmc_run(Ctx &c, int precision) {
   a.acc_off = off;
   off += c.tabs[m].P;
   a.u_off = off;
-  off += (u64)m * R;
-  a.m = m;
-  a.R = R;
-  a.P = c.P;
-  a.Y = c.d_Y.as<double>();
-  a.partial = c.d_partial.as<double>();
-  const Plan &pl = (precision == SP_PREC_FP64 && c.fast_kernel && c.fast_bo < 64) ? c.plan[1]
-                                                                               : c.plan[0];
-  a.chunks = pl.d_chunks.as<Chunk>();
-  const unsigned nchunks = (unsigned)pl.h_chunks.size();
+  off += (u64)m * c.chunk_R;
+  return off;
+}
 
+}  // namespace
+
+u64 s3ttmc_partial_slots(const Ctx &c, int precision) {
+  return plan_for(c, precision).n_partial;
+}
+
+void s3ttmc_launch(Ctx &c, int precision, const double *U, float *U32, double *Y,
+                   double *partial, bool timed) {
+  const u32 R = c.chunk_R;
+  const int m = (int)c.order - 1;
+  const u64 nU = (u64)c.dim * R;
+  const Plan &pl = plan_for(c, precision);
+  const unsigned nchunks = (unsigned)pl.h_chunks.size();
+  const bool prof = c.profile;
+  if (!timed) c.profile = false;
+  struct Restore {
+    Ctx &c;
+    bool p;
+    ~Restore() { c.profile = p; }
+  } restore{c, prof};
+
+  if (precision != SP_PREC_FP64) {
+    convert_factor<float><<<grid_for(nU, 256), 256, 0, c.stream>>>(U, nU, U32);
+    SP_CHECK_LAUNCH();
+  }
   if (precision == SP_PREC_FP64 && c.fast_kernel) {
     ScopedTimer t(c, "s3ttmc_mfma64");
     launch_fast(c.fast_kernel, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
-                pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(),
-                c.d_U.as<double>(), nullptr, nullptr, c.d_fold_ptr.as<u32>(),
-                c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
-                c.d_partial.as<double>());
+                pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(), U,
+                nullptr, nullptr, c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
+                c.d_fact.as<double>(), Y, partial);
     SP_CHECK_LAUNCH();
   } else if (precision == SP_PREC_FP32 && c.fast_kernel_f32) {
-    c.d_U32.reserve(nU * sizeof(float));
-    convert_factor<float><<<grid_for(nU, 256), 256, 0, c.stream>>>(
-        c.d_U.as<double>(), nU, c.d_U32.as<float>());
-    SP_CHECK_LAUNCH();
     ScopedTimer t(c, "s3ttmc_mfma_f32");
     launch_fast(c.fast_kernel_f32, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
                 pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
-                c.w_coef32.as<float>(), c.d_U32.as<float>(), c.d_fold_ptr.as<u32>(),
-                c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
-                c.d_partial.as<double>());
+                c.w_coef32.as<float>(), U32, c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
+                c.d_fact.as<double>(), Y, partial);
     SP_CHECK_LAUNCH();
   } else if (precision == SP_PREC_BF16 && c.fast_kernel_bf16) {
-    c.d_U32.reserve(nU * sizeof(float));
-    convert_factor<float><<<grid_for(nU, 256), 256, 0, c.stream>>>(
-        c.d_U.as<double>(), nU, c.d_U32.as<float>());
-    SP_CHECK_LAUNCH();
     ScopedTimer t(c, "s3ttmc_mfma_bf16");
     launch_fast(c.fast_kernel_bf16, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
                 pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
-                c.w_coef32.as<float>(), c.d_U32.as<float>(), c.d_fold_ptr.as<u32>(),
-                c.d_fold_term.as<u32>(), c.d_fact.as<double>(), c.d_Y.as<double>(),
-                c.d_partial.as<double>());
-    SP_CHECK_LAUNCH();
-  } else if (precision == SP_PREC_FP64) {
-    const size_t lds = off * sizeof(double);
-    if (lds > 65536) throw Error(SP_ERR_UNSUPPORTED, "packed tensors exceed LDS");
-    ScopedTimer t(c, "s3ttmc_generic");
-    s3ttmc_generic<double><<<nchunks, GEN_BLOCK, lds, c.stream>>>(
-        a, c.d_U.as<double>(), c.w_coef.as<double>());
+                c.w_coef32.as<float>(), U32, c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
+                c.d_fact.as<double>(), Y, partial);
     SP_CHECK_LAUNCH();
   } else {
-    const size_t lds = off * sizeof(float);
-    if (lds > 65536) throw Error(SP_ERR_UNSUPPORTED, "packed tensors exceed LDS");
-    c.d_U32.reserve(nU * sizeof(float));
-    convert_factor<float><<<grid_for(nU, 256), 256, 0, c.stream>>>(
-        c.d_U.as<double>(), nU, c.d_U32.as<float>());
-    SP_CHECK_LAUNCH();
-    ScopedTimer t(c, "s3ttmc_generic32");
-    s3ttmc_generic<float><<<nchunks, GEN_BLOCK, lds, c.stream>>>(
-        a, c.d_U32.as<float>(), c.w_coef32.as<float>());
-    SP_CHECK_LAUNCH();
+    GenArgs a{};
+    a.tup = c.w_tuple.as<u32>();
+    a.segptr = c.w_segptr.as<u64>();
+    a.tab_ptr = c.d_tab_ptr.as<u32>();
+    a.tab_term = c.d_tab_term.as<u32>();
+    a.fact = c.d_fact.as<double>();
+    const u64 off = generic_layout(c, a);
+    a.m = m;
+    a.R = R;
+    a.P = c.P;
+    a.Y = Y;
+    a.partial = partial;
+    a.chunks = pl.d_chunks.as<Chunk>();
+    if (precision == SP_PREC_FP64) {
+      const size_t lds = off * sizeof(double);
+      if (lds > 65536) throw Error(SP_ERR_UNSUPPORTED, "packed tensors exceed LDS");
+      ScopedTimer t(c, "s3ttmc_generic");
+      s3ttmc_generic<double><<<nchunks, GEN_BLOCK, lds, c.stream>>>(a, U,
+                                                                    c.w_coef.as<double>());
+      SP_CHECK_LAUNCH();
+    } else {
+      const size_t lds = off * sizeof(float);
+      if (lds > 65536) throw Error(SP_ERR_UNSUPPORTED, "packed tensors exceed LDS");
+      ScopedTimer t(c, "s3ttmc_generic32");
+      s3ttmc_generic<float><<<nchunks, GEN_BLOCK, lds, c.stream>>>(a, U32,
+                                                                   c.w_coef32.as<float>());
+      SP_CHECK_LAUNCH();
+    }
   }
   if (!pl.h_split.empty()) {
     ScopedTimer t(c, "reduce_split_rows");
     reduce_split_rows<<<dim3((unsigned)pl.h_split.size(), (unsigned)((c.P + 63) / 64)),
-                        64 * kSplitWaves, 0, c.stream>>>(
-        pl.d_split.as<SplitRow>(), c.d_partial.as<double>(), c.P, c.d_Y.as<double>());
+                        64 * kSplitWaves, 0, c.stream>>>(pl.d_split.as<SplitRow>(), partial,
+                                                         c.P, Y);
     SP_CHECK_LAUNCH();
   }
+}
+
+void s3ttmc_run(Ctx &c, int precision) {
+  const u32 R = c.R;
+  s3ttmc_prepare(c, R);
+  c.d_Y.reserve((u64)c.dim * c.P * sizeof(double));
+  if (precision != SP_PREC_FP64) c.d_U32.reserve((u64)c.dim * R * sizeof(float));
+  s3ttmc_launch(c, precision, c.d_U.as<double>(), c.d_U32.as<float>(), c.d_Y.as<double>(),
+                c.d_partial.as<double>(), true);
   c.y_R = R;
 }
 

@@ -100,6 +100,7 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
                        : std::strcmp(e, "tridiag-rocsolver") == 0  ? 3
                                                                    : 2;
       SP_HIP(hipGetDevice(&c.device));
+      noalloc_setup(c.device);
       SP_HIP(hipStreamCreate(&c.stream));
       c.d_inds.alloc(nnz * order * sizeof(uint32_t));
       c.d_vals.alloc(nnz * sizeof(double));
@@ -234,6 +235,16 @@ int sp_s3ttmc_dev(sp_ctx *ctx, int precision) {
     need(precision >= SP_PREC_FP64 && precision <= SP_PREC_BF16, SP_ERR_ARG,
          "unknown precision");
     s3ttmc_run(c, precision);
+  });
+}
+
+int sp_s3ttmc_noalloc(sp_ctx *ctx, const double *dU, uint32_t R, double *dY,
+                      int precision) {
+  return guard(ctx, [&] {
+    need(ctx && dU && dY, SP_ERR_ARG, "null argument");
+    need(precision >= SP_PREC_FP64 && precision <= SP_PREC_BF16, SP_ERR_ARG,
+         "unknown precision");
+    s3ttmc_noalloc(ctx->c, dU, R, dY, precision);
   });
 }
 

@@ -232,6 +232,17 @@ void radix_sort(K *keys, K *keys_tmp, u32 *vals, u32 *vals_tmp, u64 n,
 // S3TTMc (s3ttmc.hip, s3ttmc_fast.hip)
 void s3ttmc_prepare(Ctx &c, u32 R);
 void s3ttmc_run(Ctx &c, int precision);
+// Queues the S3TTMc kernels of the prepared rank (c.chunk_R) on c.stream:
+// reads the FP64 factor U (dim x R), writes its FP32 copy to U32 for the
+// reduced precisions, the packed result to Y (dim x P) and split-row partial
+// sums to `partial` (s3ttmc_partial_slots x P doubles).  `timed`: bracket the
+// launches with the context's profiling events when profiling is on.
+void s3ttmc_launch(Ctx &c, int precision, const double *U, float *U32, double *Y,
+                   double *partial, bool timed);
+u64 s3ttmc_partial_slots(const Ctx &c, int precision);
+// Allocation-free form of the launch above (noalloc.hip).
+void noalloc_setup(int device);
+void s3ttmc_noalloc(Ctx &c, const double *U, u32 R, double *Y, int precision);
 void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
 void *fast_kernel_for(int M, u32 R, int *H, int *bo, void **kf32, void **kbf16);
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
