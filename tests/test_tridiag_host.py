@@ -67,6 +67,12 @@ def cases():
     # far below / above the normal range of the pivot floor: solved scaled
     yield "tiny", 1e-250 * rng.standard_normal(40), 1e-250 * rng.standard_normal(39), 5
     yield "huge", 1e250 * rng.standard_normal(40), 1e250 * rng.standard_normal(39), 5
+    # more than 16 wanted eigenvalues: the multisection works on groups of
+    # 16 shifts over the open intervals
+    yield "random200_R40", rng.standard_normal(200), rng.standard_normal(199), 40
+    yield ("gram300_R24",) + gram_tridiagonal(rng, 800, 300, 1.0) + (24,)
+    yield ("glued_wilkinson_R20", np.concatenate([d] * 6),
+           np.concatenate([np.r_[e, 1e-10]] * 6)[:-1], 20)
 
 
 @pytest.mark.parametrize("name,d,e,R", list(cases()), ids=[c[0] for c in cases()])
