@@ -159,22 +159,56 @@ int sp_regularize(sp_ctx *ctx, double delta);
  * (symtens_norm, src/dense/symtensor.cpp:97-106). */
 int sp_core_norm(sp_ctx *ctx, double *out);
 
+/* Options of one decomposition run: the fields of Config
+ * (include/detail/tucker.hpp:14-23) plus this library's extensions.
+ * sp_tucker_opts_default fills Config's defaults (rank 2, max_iter 10,
+ * rel_tol 1e-6, delta 0.5, seed 0, init "rnd", FP64, automatic switch
+ * tolerance, no subspace test, silent). */
+typedef struct sp_tucker_opts {
+  int alg;             /* 0 TuckerHOOI, 1 TuckerHOQRI, 2 TuckerAltHOOI       */
+  uint32_t rank;
+  uint64_t max_iter;
+  double rel_tol;      /* stop when |error - previous error| < rel_tol     */
+  double delta;        /* row clip of regularize                           */
+  uint64_t seed;       /* random initial factor (std::mt19937 + N(0,1))    */
+  const char *init;    /* "rnd" or "hoevd" (NULL: "rnd"); U0 overrides     */
+  int precision;       /* SP_PREC_* of the S3TTMc iterations               */
+  double switch_tol;   /* reduced forms: |error change| that ends the
+                          reduced phase (<= 0: 1e-3 BF16, 1e-5 FP32)        */
+  double subspace_tol; /* > 0: stop instead when the sine of the largest
+                          principal angle between consecutive factors is
+                          below it                                         */
+  int verbose;         /* per-iteration lines on stdout                    */
+} sp_tucker_opts;
+
+void sp_tucker_opts_default(sp_tucker_opts *o);
+
 /* One decomposition run of the host drop-in layer on a context:
- * alg 0 = SpSymtensor::TuckerHOOI, 1 = TuckerHOQRI, 2 = TuckerAltHOOI
+ * SpSymtensor::TuckerHOOI / TuckerHOQRI / TuckerAltHOOI
  * (include/detail/sparse/spsymtensor.hpp:40-42, src/sparse/spsymtensor.cpp:
- * 201-337) with the fields of Config (include/detail/tucker.hpp:14-23).
- * init: "rnd" or "hoevd"; U0 (dim x rank, optional) overrides it.
- * precision: SP_PREC_* of the S3TTMc iterations; reduced forms switch to
- * FP64 once |error change| < switch_tol (<= 0: 1e-3 for BF16, 1e-5 for
- * FP32) and then iterate in FP64 until |error change| < rel_tol.
- * Outputs: U_out (dim x rank), errors (max_iter entries, may be null), the
- * iteration count, the count of FP64 iterations after the switch and the
- * wall time of the loop. */
-int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
-                  double rel_tol, double delta, uint64_t seed, const char *init,
-                  int precision, double switch_tol, const double *U0,
-                  int verbose, double *U_out, double *errors, uint64_t *iters,
-                  uint64_t *fp64_sweeps, double *seconds);
+ * 201-337).  U0 (dim x rank, optional) replaces the initial factor.  The
+ * reduced-precision forms run their S3TTMc in FP32/BF16 until the error
+ * changes by less than switch_tol, then in FP64; a stopping test compares
+ * two FP64 iterations only (the first FP64 iteration after the switch never
+ * stops the run).
+ * Outputs: U_out (dim x rank); per iteration (max_iter entries each, any may
+ * be NULL): errors, core_norms (||C||, the packed weighted norm) and times
+ * (seconds from the start of the run, initialisation included, to the end
+ * of the iteration); the
+ * iteration count, the FP64 iterations after the switch and the wall time
+ * of the loop. */
+int sp_tucker_run(sp_ctx *ctx, const sp_tucker_opts *opts, const double *U0,
+                  double *U_out, double *errors, double *core_norms, double *times,
+                  uint64_t *iters, uint64_t *fp64_sweeps, double *seconds);
+
+/* Subspace movement of the factor, for convergence tests on the factor
+ * rather than on the fit (no counterpart in the reference, whose loops stop
+ * on |error change| only, src/sparse/spsymtensor.cpp:235-238).
+ * sp_factor_snapshot keeps a device copy of the current U; sp_subspace_sin
+ * gives the sine of the largest principal angle between span(U) and the
+ * snapshot's span (both orthonormal; resolution about 1e-8). */
+int sp_factor_snapshot(sp_ctx *ctx);
+int sp_subspace_sin(sp_ctx *ctx, double *out);
 
 /* ---- HOEVD initialisation (replaces hosvd_init's "hoevd" branch,
  * src/sparse/spsymtensor.cpp:363-382, with sparse_syrk 385-409 and

@@ -82,10 +82,11 @@ SIGNATURES = {
                                     ctypes.c_uint64, ctypes.c_float, ctypes.c_uint64, f32p,
                                     f64p]),
     "sp_mlp_predict": (ctypes.c_int, [vp, f32p, ctypes.c_uint64, f32p]),
-    "sp_tucker_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
-                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
-                                     ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
-                                     f64p, ctypes.c_int, f64p, f64p, u64p, u64p, f64p]),
+    "sp_tucker_opts_default": (None, [ctypes.c_void_p]),
+    "sp_tucker_run": (ctypes.c_int, [vp, vp, f64p, f64p, f64p, f64p, f64p, u64p, u64p,
+                                     f64p]),
+    "sp_factor_snapshot": (ctypes.c_int, [vp]),
+    "sp_subspace_sin": (ctypes.c_int, [vp, f64p]),
     "sp_timing_enable": (ctypes.c_int, [vp, ctypes.c_int]),
     "sp_timing_get": (ctypes.c_int, [vp, ctypes.c_char_p, f64p, u64p]),
     "sp_timing_reset": (ctypes.c_int, [vp]),
@@ -97,6 +98,16 @@ SIGNATURES = {
 }
 
 _LIB = None
+
+
+class TuckerOpts(ctypes.Structure):
+    """sp_tucker_opts (include/symprop_amd.h)."""
+    _fields_ = [("alg", ctypes.c_int), ("rank", ctypes.c_uint32),
+                ("max_iter", ctypes.c_uint64), ("rel_tol", ctypes.c_double),
+                ("delta", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("init", ctypes.c_char_p), ("precision", ctypes.c_int),
+                ("switch_tol", ctypes.c_double), ("subspace_tol", ctypes.c_double),
+                ("verbose", ctypes.c_int)]
 
 
 class SpError(RuntimeError):
@@ -308,26 +319,46 @@ class SpSymtensor:
 
     # decomposition -------------------------------------------------------
     def tucker(self, alg=0, rank=2, max_iter=10, rel_tol=1e-6, delta=0.5, seed=0,
-               init="rnd", precision=PREC_FP64, switch_tol=-1.0, U0=None, verbose=False):
+               init="rnd", precision=PREC_FP64, switch_tol=-1.0, U0=None, verbose=False,
+               subspace_tol=0.0):
         """One TuckerHOOI (alg 0) / TuckerHOQRI (1) / TuckerAltHOOI (2) run.
 
-        Returns dict(U, errors, iters, fp64_sweeps, seconds).
+        Returns dict(U, errors, core_norms, times, iters, fp64_sweeps, seconds);
+        errors, core_norms and times (seconds since the run started) are per
+        iteration.
         """
+        o = TuckerOpts()
+        lib().sp_tucker_opts_default(ctypes.byref(o))
+        init_b = init.encode()
+        o.alg, o.rank, o.max_iter, o.rel_tol, o.delta, o.seed = (
+            alg, rank, max_iter, rel_tol, delta, seed)
+        o.init, o.precision, o.switch_tol, o.subspace_tol = (
+            init_b, precision, switch_tol, subspace_tol)
+        o.verbose = 1 if verbose else 0
         U = np.zeros((self.dim, rank))
-        errs = np.zeros(max(1, max_iter))
+        m = max(1, max_iter)
+        errs, cns, ts = np.zeros(m), np.zeros(m), np.zeros(m)
         it, sw, sec = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
         u0 = None
         if U0 is not None:
             u0 = np.ascontiguousarray(U0, dtype=np.float64)
             assert u0.shape == (self.dim, rank)
-        _check(lib().sp_tucker_run(self.h, alg, rank, max_iter, rel_tol, delta, seed,
-                                   init.encode(), precision, switch_tol,
+        _check(lib().sp_tucker_run(self.h, ctypes.byref(o),
                                    _p(u0, f64p) if u0 is not None else None,
-                                   1 if verbose else 0, _p(U, f64p), _p(errs, f64p),
+                                   _p(U, f64p), _p(errs, f64p), _p(cns, f64p), _p(ts, f64p),
                                    ctypes.byref(it), ctypes.byref(sw), ctypes.byref(sec)))
         self.R = rank
-        return dict(U=U, errors=errs[: it.value].copy(), iters=it.value,
-                    fp64_sweeps=sw.value, seconds=sec.value)
+        k = it.value
+        return dict(U=U, errors=errs[:k].copy(), core_norms=cns[:k].copy(),
+                    times=ts[:k].copy(), iters=k, fp64_sweeps=sw.value, seconds=sec.value)
+
+    def factor_snapshot(self):
+        _check(lib().sp_factor_snapshot(self.h))
+
+    def subspace_sin(self):
+        v = ctypes.c_double()
+        _check(lib().sp_subspace_sin(self.h, ctypes.byref(v)))
+        return v.value
 
     # timing ------------------------------------------------------------
     def timing(self, on=True):

@@ -820,6 +820,40 @@ void dense_regularize(Ctx &c, double delta) {
   orthonormalize_gram(c, c.ws_nr, true);
 }
 
+void dense_factor_snapshot(Ctx &c) {
+  const u64 bytes = (u64)c.dim * c.R * sizeof(double);
+  c.ws_uprev.reserve(bytes);
+  SP_HIP(hipMemcpyAsync(c.ws_uprev.p, c.d_U.p, bytes, hipMemcpyDeviceToDevice, c.stream));
+  c.uprev_R = c.R;
+}
+
+// sin of the largest principal angle between span(U) and span(U_snapshot),
+// both with orthonormal columns: sqrt(1 - sigma_min(U^T U_snapshot)^2), with
+// sigma_min^2 the smallest eigenvalue of M^T M, M = U^T U_snapshot (R x R,
+// device product, host Jacobi).  Resolution about 1e-8 (the 1 - c^2 form).
+double dense_subspace_sin(Ctx &c) {
+  const u32 R = c.R;
+  DevBuf &g = c.ws_rr;
+  g.reserve((u64)R * R * sizeof(double));
+  atb(c, c.d_U.as<double>(), R, c.ws_uprev.as<double>(), R, c.dim, g.as<double>(),
+      "subspace_atb");
+  std::vector<double> M((size_t)R * R);
+  SP_HIP(hipMemcpyAsync(M.data(), g.p, M.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> MtM((size_t)R * R, 0.0);
+  for (u32 i = 0; i < R; ++i)
+    for (u32 j = 0; j < R; ++j) {
+      double v = 0.0;
+      for (u32 k = 0; k < R; ++k) v += M[(size_t)k * R + i] * M[(size_t)k * R + j];
+      MtM[(size_t)i * R + j] = v;
+    }
+  std::vector<double> w, V;
+  jacobi_eig(MtM, (int)R, w, V);
+  const double smin2 = std::min(1.0, std::max(0.0, w[R - 1]));
+  return std::sqrt(1.0 - smin2);
+}
+
 double dense_core_norm(Ctx &c) {
   std::vector<double> core((u64)c.R * c.P), W(c.P);
   SP_HIP(hipMemcpyAsync(core.data(), c.d_core.p, core.size() * sizeof(double),

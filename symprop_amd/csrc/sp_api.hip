@@ -353,6 +353,24 @@ int sp_core_norm(sp_ctx *ctx, double *out) {
   });
 }
 
+int sp_factor_snapshot(sp_ctx *ctx) {
+  return guard(ctx, [&] {
+    need(ctx, SP_ERR_ARG, "null context");
+    need_factor(ctx->c);
+    dense_factor_snapshot(ctx->c);
+  });
+}
+
+int sp_subspace_sin(sp_ctx *ctx, double *out) {
+  return guard(ctx, [&] {
+    need(ctx && out, SP_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    need_factor(c);
+    need(c.ws_uprev.p && c.uprev_R == c.R, SP_ERR_STATE, "no factor snapshot of this rank");
+    *out = dense_subspace_sin(c);
+  });
+}
+
 int sp_hoevd_gram(sp_ctx *ctx, double *A) {
   return guard(ctx, [&] {
     need(ctx && A, SP_ERR_ARG, "null argument");

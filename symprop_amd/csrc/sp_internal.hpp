@@ -178,6 +178,8 @@ struct Ctx {
   DevBuf ws_rr;    // R x R
   DevBuf ws_eig;   // eigen-step vectors (D, E, tau, R top vectors) + info words
   DevBuf ws_tiles; // Gram tile list
+  DevBuf ws_uprev; // factor snapshot (n x uprev_R) for subspace angles
+  u32 uprev_R = 0;
   int eig_method = 2;  // 2 (default): own tridiagonal reduction (rocSOLVER sytrd
                        // beyond its LDS limit) + host top-R tridiagonal solve +
                        // rocSOLVER ormtr; 3: as 2 with rocSOLVER sytrd always;
@@ -258,6 +260,8 @@ void dense_leftsvd_y(Ctx &c, double *sigma);
 void dense_qr(Ctx &c);
 void dense_regularize(Ctx &c, double delta);
 double dense_core_norm(Ctx &c);
+void dense_factor_snapshot(Ctx &c);
+double dense_subspace_sin(Ctx &c);
 
 // HOEVD initial factor (hoevd.hip)
 void hoevd_gram(Ctx &c, double *A);

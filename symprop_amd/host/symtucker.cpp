@@ -88,26 +88,51 @@ real_t auto_switch_tol(const Config &c) {
   return std::max(t, c.rel_tol);
 }
 
-// Precision of the next S3TTMc in the mixed forms: reduced until the error
-// change falls below the switch tolerance, FP64 afterwards.
-struct PrecisionSchedule {
-  int prec;
-  bool low;
-  real_t switch_tol;
-  index_t fp64 = 0;
-  explicit PrecisionSchedule(const Config &c)
-      : prec(c.precision), low(c.precision != SP_PREC_FP64),
-        switch_tol(auto_switch_tol(c)) {}
-  int current() const { return low ? prec : SP_PREC_FP64; }
-  // Returns true when the loop may stop (FP64 phase and converged).
-  bool step(real_t diff, real_t rel_tol) {
-    if (low) {
-      if (diff < switch_tol) low = false;
+// Precision schedule and stopping test of the decomposition loops.  The
+// reference stops when the error differs from the previous iteration's by
+// less than rel_tol (spsymtensor.cpp:235-238).  The reduced-precision forms
+// run S3TTMc in FP32/BF16 until the error changes by less than the switch
+// tolerance, then in FP64; the test only ever compares two FP64 iterations,
+// so the first FP64 iteration after the switch never stops the run.  With
+// subspace_tol > 0 the test is on the factor instead: the sine of the
+// largest principal angle between consecutive factors.
+class IterationControl {
+public:
+  IterationControl(const Config &c, sp_ctx *ctx)
+      : _ctx(ctx), _prec(c.precision), _low(c.precision != SP_PREC_FP64),
+        _switch_tol(auto_switch_tol(c)), _rel_tol(c.rel_tol), _sub_tol(c.subspace_tol) {}
+
+  int precision() const { return _low ? _prec : SP_PREC_FP64; }
+  index_t fp64_sweeps() const { return _fp64; }
+
+  // Called once the iteration's factor is final; returns true to stop.
+  bool finish(real_t error_diff) {
+    real_t moved = 1.0;
+    if (_sub_tol > 0) {
+      if (_have_snapshot) ck(sp_subspace_sin(_ctx, &moved));
+      ck(sp_factor_snapshot(_ctx));
+      _have_snapshot = true;
+    }
+    if (_low) {
+      if (error_diff < _switch_tol) _low = false;
+      _prev_fp64 = false;
       return false;
     }
-    if (prec != SP_PREC_FP64) ++fp64;
-    return diff < rel_tol;
+    if (_prec != SP_PREC_FP64) ++_fp64;
+    const bool comparable = _prec == SP_PREC_FP64 || _prev_fp64;
+    _prev_fp64 = true;
+    if (!comparable) return false;
+    return _sub_tol > 0 ? moved < _sub_tol : error_diff < _rel_tol;
   }
+
+private:
+  sp_ctx *_ctx;
+  int _prec;
+  bool _low;
+  real_t _switch_tol, _rel_tol, _sub_tol;
+  index_t _fp64 = 0;
+  bool _prev_fp64 = false;
+  bool _have_snapshot = false;
 };
 
 template <class F>
@@ -350,28 +375,29 @@ SymTuckerDecomp SpSymtensor::TuckerHOOI(const Config &c) const {
   const size_t full = (size_t)std::pow(R, _order - 1);
   std::vector<real_t> core_p((size_t)R * P);
   ck(sp_factor_upload(_ctx, factor.data(), R));
-  PrecisionSchedule sched(c);
+  IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
   for (size_t iter = 0; iter < c.max_iter; iter++) {
     timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
-    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, sched.current())); });
+    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
     timed("leftsvd", _ctx, [&] { ck(sp_leftsvd_y(_ctx, nullptr)); });
     timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
     for (dim_t r = 0; r < R; ++r)
       unpack_symtens(core_p.data() + (size_t)r * P, R, _order - 1, core.data() + r * full);
     const real_t rel_res = rel_residual(core_prev, core);
-    const real_t error = decomp.reconstruct_error(*this);
+    const real_t core_norm = plain_norm(core);
+    const real_t error = fit_error(_norm, core_norm);  // reconstruct_error
     if (c.verbose)
       std::cout << std::scientific << std::setprecision(12) << "Iteration: " << iter
                 << " relative residual: " << rel_res << " error " << error << "\n";
-    decomp._errors.push_back(error);
+    decomp.record(error, core_norm, t0);
     const real_t error_diff = std::abs(prev_error - error);
-    if (sched.step(error_diff, c.rel_tol)) break;
+    if (ctl.finish(error_diff)) break;
     prev_error = error;
     core_prev = core;
   }
   ck(sp_factor_download(_ctx, factor.data()));
-  decomp._fp64_sweeps = sched.fp64;
+  decomp._fp64_sweeps = ctl.fp64_sweeps();
   decomp._seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return decomp;
@@ -389,11 +415,11 @@ SymTuckerDecomp SpSymtensor::TuckerHOQRI(const Config &c) const {
   std::vector<real_t> core((size_t)R * ncols), core_prev((size_t)R * ncols);
   const std::vector<real_t> coeff = symtens_coeff(R, _order - 1);
   ck(sp_factor_upload(_ctx, factor.data(), R));
-  PrecisionSchedule sched(c);
+  IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
   for (size_t iter = 0; iter < c.max_iter; iter++) {
     timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
-    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, sched.current())); });
+    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
     timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core.data())); });
     timed("symgemm", _ctx, [&] { ck(sp_symgemm(_ctx)); });
     timed("qr", _ctx, [&] { ck(sp_qr(_ctx)); });
@@ -403,10 +429,10 @@ SymTuckerDecomp SpSymtensor::TuckerHOQRI(const Config &c) const {
     if (c.verbose)
       std::cout << std::scientific << std::setprecision(12) << "Iteration: " << iter
                 << " relative residual: " << rel_res << " error " << error << "\n";
-    decomp._errors.push_back(error);
+    decomp.record(error, core_norm, t0);
     core_prev = core;
     const real_t error_diff = std::abs(prev_error - error);
-    if (sched.step(error_diff, c.rel_tol)) break;
+    if (ctl.finish(error_diff)) break;
     prev_error = error;
   }
   const size_t unpacked = (size_t)std::pow(R, _order - 1);
@@ -414,7 +440,7 @@ SymTuckerDecomp SpSymtensor::TuckerHOQRI(const Config &c) const {
     unpack_symtens(core.data() + (size_t)i * ncols, R, _order - 1,
                    decomp._core.data() + (size_t)i * unpacked);
   ck(sp_factor_download(_ctx, factor.data()));
-  decomp._fp64_sweeps = sched.fp64;
+  decomp._fp64_sweeps = ctl.fp64_sweeps();
   decomp._seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return decomp;
@@ -436,11 +462,11 @@ SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
   const size_t ncols = symtens_size(R, _order - 1);
   std::vector<real_t> core_p((size_t)R * ncols);
   ck(sp_factor_upload(_ctx, factor.data(), R));
-  PrecisionSchedule sched(c);
+  IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
   for (size_t iter = 0; iter < c.max_iter; iter++) {
     timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
-    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, sched.current())); });
+    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
     timed("syevd", _ctx, [&] {
       ck(sp_leftsvd_y(_ctx, nullptr));
       ck(sp_factor_download(_ctx, factor.data()));
@@ -456,13 +482,13 @@ SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
     if (c.verbose)
       std::cout << std::scientific << std::setprecision(10) << "Iteration: " << iter
                 << " relative residual: " << rel_res << " error " << error << "\n";
-    decomp._errors.push_back(error);
+    decomp.record(error, core_norm, t0);
     const real_t error_diff = std::abs(prev_error - error);
-    if (sched.step(error_diff, c.rel_tol)) break;
+    if (ctl.finish(error_diff)) break;
     prev_error = error;
     factor_prev = factor;
   }
-  decomp._fp64_sweeps = sched.fp64;
+  decomp._fp64_sweeps = ctl.fp64_sweeps();
   decomp._seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return decomp;
@@ -471,35 +497,55 @@ SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
 }  // namespace symprop
 
 // ------------------------------------------------------------ C entry
-extern "C" int sp_tucker_run(sp_ctx *ctx, int alg, uint32_t rank, uint64_t max_iter,
-                             double rel_tol, double delta, uint64_t seed, const char *init,
-                             int precision, double switch_tol, const double *U0,
-                             int verbose, double *U_out, double *errors,
+extern "C" void sp_tucker_opts_default(sp_tucker_opts *o) {
+  if (!o) return;
+  const symprop::Config c;
+  o->alg = 0;
+  o->rank = c.rank;
+  o->max_iter = c.max_iter;
+  o->rel_tol = c.rel_tol;
+  o->delta = c.delta;
+  o->seed = c.seed;
+  o->init = "rnd";
+  o->precision = c.precision;
+  o->switch_tol = c.switch_tol;
+  o->subspace_tol = c.subspace_tol;
+  o->verbose = 0;
+}
+
+extern "C" int sp_tucker_run(sp_ctx *ctx, const sp_tucker_opts *o, const double *U0,
+                             double *U_out, double *errors, double *core_norms, double *times,
                              uint64_t *iters, uint64_t *fp64_sweeps, double *seconds) {
   using namespace symprop;
   try {
-    if (!ctx || !U_out || !iters) throw spamd::Error(SP_ERR_ARG, "null argument");
-    if (alg < 0 || alg > 2) throw spamd::Error(SP_ERR_ARG, "alg must be 0, 1 or 2");
-    if (rank < 1) throw spamd::Error(SP_ERR_ARG, "rank must be >= 1");
+    if (!ctx || !o || !U_out || !iters) throw spamd::Error(SP_ERR_ARG, "null argument");
+    if (o->alg < 0 || o->alg > 2) throw spamd::Error(SP_ERR_ARG, "alg must be 0, 1 or 2");
+    if (o->rank < 1) throw spamd::Error(SP_ERR_ARG, "rank must be >= 1");
+    if (o->precision < SP_PREC_FP64 || o->precision > SP_PREC_BF16)
+      throw spamd::Error(SP_ERR_ARG, "unknown precision");
     SpSymtensor T(ctx);
     Config c;
-    c.rank = rank;
-    c.max_iter = max_iter;
-    c.rel_tol = rel_tol;
-    c.delta = delta;
-    c.seed = seed;
-    c.init = init ? init : "rnd";
-    c.precision = precision;
-    c.switch_tol = switch_tol;
-    c.verbose = verbose != 0;
+    c.rank = o->rank;
+    c.max_iter = o->max_iter;
+    c.rel_tol = o->rel_tol;
+    c.delta = o->delta;
+    c.seed = o->seed;
+    c.init = o->init ? o->init : "rnd";
+    c.precision = o->precision;
+    c.switch_tol = o->switch_tol;
+    c.subspace_tol = o->subspace_tol;
+    c.verbose = o->verbose != 0;
     if (U0) {
       c.init = "given";
-      c.U0.assign(U0, U0 + (size_t)T.dim() * rank);
+      c.U0.assign(U0, U0 + (size_t)T.dim() * o->rank);
     }
-    const SymTuckerDecomp r =
-        alg == 0 ? T.TuckerHOOI(c) : alg == 1 ? T.TuckerHOQRI(c) : T.TuckerAltHOOI(c);
+    const SymTuckerDecomp r = o->alg == 0   ? T.TuckerHOOI(c)
+                              : o->alg == 1 ? T.TuckerHOQRI(c)
+                                            : T.TuckerAltHOOI(c);
     std::copy(r.factor().begin(), r.factor().end(), U_out);
     if (errors) std::copy(r.errors().begin(), r.errors().end(), errors);
+    if (core_norms) std::copy(r.core_norms().begin(), r.core_norms().end(), core_norms);
+    if (times) std::copy(r.times().begin(), r.times().end(), times);
     *iters = r.iterations();
     if (fp64_sweeps) *fp64_sweeps = r.fp64_sweeps();
     if (seconds) *seconds = r.seconds();

@@ -38,6 +38,9 @@ struct Config {
   int precision = 0;            // SP_PREC_FP64 / SP_PREC_FP32 / SP_PREC_BF16
   real_t switch_tol = -1.0;     // mixed forms: |d error| that ends the
                                 // reduced-precision phase (<0: automatic)
+  real_t subspace_tol = 0.0;    // > 0: stop when the factor's subspace moves
+                                // by less (sine of the largest principal
+                                // angle) instead of on the error change
   bool verbose = true;          // per-iteration lines on stdout
   std::vector<real_t> U0;       // initial factor for init == "given"
 
@@ -55,6 +58,8 @@ public:
   const std::vector<real_t> &factor() const { return _factor; }
   const std::vector<real_t> &core() const { return _core; }
   const std::vector<real_t> &errors() const { return _errors; }
+  const std::vector<real_t> &core_norms() const { return _core_norms; }
+  const std::vector<double> &times() const { return _times; }
   index_t iterations() const { return _errors.size(); }
   index_t fp64_sweeps() const { return _fp64_sweeps; }
   double seconds() const { return _seconds; }
@@ -63,6 +68,12 @@ public:
 
 private:
   SymTuckerDecomp(const Config &config, dim_t dim, order_t order);
+  void record(real_t error, real_t core_norm, std::chrono::steady_clock::time_point t0) {
+    _errors.push_back(error);
+    _core_norms.push_back(core_norm);
+    _times.push_back(
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  }
 
   dim_t _dim;
   dim_t _rank;
@@ -71,6 +82,8 @@ private:
   std::vector<real_t> _core;
   std::vector<real_t> _factor;
   std::vector<real_t> _errors;
+  std::vector<real_t> _core_norms;
+  std::vector<double> _times;  // seconds from the loop start to each iteration's end
   index_t _fp64_sweeps = 0;
   double _seconds = 0.0;
 };
