@@ -6,7 +6,9 @@ module restates, in FP64 numpy, exactly what the library's HIP kernels
 compute, so that tests can check them:
 
   node features   64 per node: log(1+degree), log(1+distinct partners),
-                  and a 62-dim sketch: fixed hash-seeded Gaussian node
+                  log(1+max pair multiplicity), log(1+repeated partners)
+                  (multiplicity of {i, j}: its occurrences over all position
+                  pairs of all nonzeros), and a 60-dim sketch: fixed hash-seeded Gaussian node
                   vectors propagated `rounds` times through the owner lists
                   (s'[r] = sum over owners (leaf L, rem r, value v) of
                   v * sum_{a in L} s[a]) with row L2 normalisation after
@@ -21,7 +23,7 @@ import numpy as np
 
 import oracle
 
-SKETCH = 62
+SKETCH = 60
 
 
 def splitmix(x):
@@ -72,6 +74,20 @@ def node_features(order, dim, inds, vals, rounds=3, seed=12345):
     for a, b in pairs:
         partners[a] += 1
         partners[b] += 1
+    mult = {}
+    for a in range(order):
+        for b in range(a + 1, order):
+            for x, y in t[:, [a, b]]:
+                if x != y:
+                    mult[(x, y)] = mult.get((x, y), 0) + 1
+    maxmult = np.zeros(dim)
+    rep = np.zeros(dim)
+    for (x, y), m in mult.items():
+        maxmult[x] = max(maxmult[x], m)
+        maxmult[y] = max(maxmult[y], m)
+        if m >= 2:
+            rep[x] += 1
+            rep[y] += 1
     s = node_gauss(dim, SKETCH, seed)
     for _ in range(rounds):
         agg = s[leaf].sum(axis=1) * v[:, None]
@@ -79,7 +95,8 @@ def node_features(order, dim, inds, vals, rounds=3, seed=12345):
         np.add.at(new, rem, agg)
         nrm = np.sqrt((new ** 2).sum(axis=1, keepdims=True))
         s = np.where(nrm > 0, new / np.where(nrm > 0, nrm, 1.0), 0.0)
-    return np.concatenate([np.log1p(deg)[:, None], np.log1p(partners)[:, None], s], axis=1)
+    return np.concatenate([np.log1p(deg)[:, None], np.log1p(partners)[:, None],
+                           np.log1p(maxmult)[:, None], np.log1p(rep)[:, None], s], axis=1)
 
 
 def param_shapes(din=64, dh=256, dout=8):

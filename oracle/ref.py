@@ -31,6 +31,9 @@ def lib():
         L.ref_build.restype = ctypes.c_void_p
         L.ref_build.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                 u32p, f64p]
+        L.ref_build_quiet.restype = ctypes.c_void_p
+        L.ref_build_quiet.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                      u32p, f64p]
         L.ref_free.argtypes = [ctypes.c_void_p]
         L.ref_dump.restype = ctypes.c_char_p
         L.ref_dump.argtypes = [ctypes.c_void_p]

@@ -56,6 +56,22 @@ RefTree *ref_build(uint32_t order, uint32_t dim, uint64_t nnz,
   return h;
 }
 
+// The same construction without the print() dump: what SpSymtensor's
+// constructor does (src/sparse/spsymtensor.cpp:16-28), for timing it.
+RefTree *ref_build_quiet(uint32_t order, uint32_t dim, uint64_t nnz,
+                         const uint32_t *inds, const double *vals) {
+  auto *h = new RefTree();
+  h->order = order;
+  h->dim = dim;
+  DTree dt;
+  for (uint64_t i = 0; i < nnz; ++i) {
+    dt.banerjee_insert(std::span<const dim_t>(inds + i * order, order),
+                       vals[i]);
+  }
+  h->cdt = std::make_unique<CompressedDTree>(dt, order - 1);
+  return h;
+}
+
 void ref_free(RefTree *h) { delete h; }
 
 // Text of CompressedDTree::print() for the tree built by ref_build.

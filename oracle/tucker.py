@@ -46,7 +46,7 @@ def rel_residual(x, y):
     return ((x - y) ** 2).sum() / (x ** 2).sum()
 
 
-def hooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
+def hooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None, nthreads=4):
     N, n = T.order, T.dim
     norm = oracle.norm(N, T.inds, T.vals)
     U = oracle.random_factor(seed, n, R) if U0 is None else np.array(U0, float)
@@ -55,7 +55,7 @@ def hooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
     prev = 0.0
     for _ in range(max_iter):
         U = regularize(U, delta)
-        Y = unpack_rows(T.s3ttmc(U, R, nthreads=4), R, N - 1)
+        Y = unpack_rows(T.s3ttmc(U, R, nthreads=nthreads), R, N - 1)
         U = np.linalg.svd(Y, full_matrices=False)[0][:, :R]
         core = U.T @ Y
         err = fit_error(norm, np.sqrt((core ** 2).sum()))
@@ -67,7 +67,7 @@ def hooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
     return U, np.array(errors), core
 
 
-def hoqri(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
+def hoqri(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None, nthreads=4):
     N, n = T.order, T.dim
     norm = oracle.norm(N, T.inds, T.vals)
     W = oracle.symtens_coeff(R, N - 1)
@@ -77,7 +77,7 @@ def hoqri(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
     core = None
     for _ in range(max_iter):
         U = regularize(U, delta)
-        Y = T.s3ttmc(U, R, nthreads=4)
+        Y = T.s3ttmc(U, R, nthreads=nthreads)
         core = U.T @ Y
         U = np.linalg.qr((Y * W) @ core.T)[0]
         cn = np.sqrt((core ** 2 * W).sum())
@@ -89,7 +89,7 @@ def hoqri(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
     return U, np.array(errors), core
 
 
-def althooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
+def althooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None, nthreads=4):
     N, n = T.order, T.dim
     norm = oracle.norm(N, T.inds, T.vals)
     W = oracle.symtens_coeff(R, N - 1)
@@ -99,7 +99,7 @@ def althooi(T, R, max_iter, rel_tol=1e-6, delta=0.5, seed=0, U0=None):
     core = None
     for _ in range(max_iter):
         U = regularize(U, delta)
-        Y = T.s3ttmc(U, R, nthreads=4)
+        Y = T.s3ttmc(U, R, nthreads=nthreads)
         A = (Y * W) @ Y.T
         U = np.linalg.eigh(A)[1][:, n - R:]
         core = U.T @ Y

@@ -6,7 +6,10 @@
 // Features (64 per node), computed from the tensor's owner lists:
 //   [0]  log(1 + degree)              degree = owners with rem = node
 //   [1]  log(1 + distinct partners)   from the level-1 pairs of the tree
-//   [2:] 62-dim sketch: hash-seeded Gaussian node vectors propagated
+//   [2]  log(1 + max pair multiplicity): over the node's partners j, the
+//        largest number of (nonzero, position pair) occurrences of {i, j}
+//   [3]  log(1 + repeated partners): partners j with multiplicity >= 2
+//   [4:] 60-dim sketch: hash-seeded Gaussian node vectors propagated
 //        `rounds` times, s'[r] = sum over owners (L, r, v) of v*sum_{a in L} s[a],
 //        L2-normalised per row after each round (FP64, stored as FP32).
 // MLP: din -> dh -> dh -> dout, ReLU, row-major W[in][out]; loss = mean over
@@ -21,7 +24,7 @@ namespace spamd {
 
 namespace {
 
-constexpr int SKETCH = 62;
+constexpr int SKETCH = 60;
 constexpr int NFEAT = 64;
 
 __host__ __device__ inline u64 mix64(u64 x) {
@@ -111,8 +114,46 @@ __global__ void partner_counts(const u64 *__restrict__ keys, u64 L, int b,
   }
 }
 
+// Every position pair (a < b) of every nonzero as a packed key t[a] << b | t[b]
+// (1-based indices, sorted tuples); a pair of equal indices gets `dead`,
+// the key of (2^b - 1, 2^b - 1), which no pair of distinct indices has.
+__global__ void pair_keys(const u32 *__restrict__ inds, u64 nnz, int N, int b, u64 dead,
+                          u64 *__restrict__ keys) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const u32 *t = inds + i * N;
+  u64 o = i * (u64)(N * (N - 1) / 2);
+  for (int a = 0; a < N; ++a)
+    for (int c = a + 1; c < N; ++c)
+      keys[o++] = t[a] == t[c] ? dead : ((u64)t[a] << b) | t[c];
+}
+
+// Runs of equal keys in the sorted pair keys: the first element of each run
+// measures the run and raises both nodes' maximum multiplicity and, for a
+// run of at least 2, both nodes' count of repeated partners (integer
+// atomics: the result does not depend on their order).
+__global__ void pair_runs(const u64 *__restrict__ keys, u64 L, int b, u64 dead,
+                          unsigned int *__restrict__ maxmult, unsigned int *__restrict__ rep) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L) return;
+  const u64 k = keys[i];
+  if (k == dead || (i > 0 && keys[i - 1] == k)) return;
+  u64 j = i + 1;
+  while (j < L && keys[j] == k) ++j;
+  const unsigned int m = (unsigned int)min<u64>(j - i, 0xffffffffull);
+  const u32 lo = (u32)(k & ((1ull << b) - 1)), hi = (u32)(k >> b);
+  atomicMax(&maxmult[hi - 1], m);
+  atomicMax(&maxmult[lo - 1], m);
+  if (m >= 2) {
+    atomicAdd(&rep[hi - 1], 1u);
+    atomicAdd(&rep[lo - 1], 1u);
+  }
+}
+
 __global__ void assemble_features(u32 n, const u64 *__restrict__ segptr,
                                   const unsigned int *__restrict__ partners,
+                                  const unsigned int *__restrict__ maxmult,
+                                  const unsigned int *__restrict__ rep,
                                   const double *__restrict__ s, float *__restrict__ f) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (u64)n * NFEAT) return;
@@ -121,7 +162,9 @@ __global__ void assemble_features(u32 n, const u64 *__restrict__ segptr,
   double v;
   if (k == 0) v = log1p((double)(segptr[i + 1] - segptr[i]));
   else if (k == 1) v = log1p((double)partners[i]);
-  else v = s[i * SKETCH + (k - 2)];
+  else if (k == 2) v = log1p((double)maxmult[i]);
+  else if (k == 3) v = log1p((double)rep[i]);
+  else v = s[i * SKETCH + (k - 4)];
   f[t] = (float)v;
 }
 
@@ -511,9 +554,34 @@ void node_features(Ctx &c, int rounds, u64 seed, float *f) {
         l1.keys.as<u64>(), l1.size, c.bits, partners.as<unsigned int>());
     SP_CHECK_LAUNCH();
   }
+  // pair multiplicities: sort every position pair of every nonzero
+  DevBuf maxmult((u64)n * sizeof(unsigned int)), rep((u64)n * sizeof(unsigned int));
+  SP_HIP(hipMemsetAsync(maxmult.p, 0, (u64)n * sizeof(unsigned int), c.stream));
+  SP_HIP(hipMemsetAsync(rep.p, 0, (u64)n * sizeof(unsigned int), c.stream));
+  {
+    const int N = (int)c.order, b = c.bits;
+    if (2 * b > 64) throw Error(SP_ERR_UNSUPPORTED, "pair keys need 2*bits(dim) <= 64");
+    const u64 np2 = c.nnz * (u64)(N * (N - 1) / 2);
+    const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1);
+    const u64 dead = (mask << b) | mask;
+    DevBuf ka(np2 * sizeof(u64)), kb(np2 * sizeof(u64));
+    pair_keys<<<grid_for(c.nnz, 256), 256, 0, c.stream>>>(c.d_inds.as<u32>(), c.nnz, N, b,
+                                                          dead, ka.as<u64>());
+    SP_CHECK_LAUNCH();
+    u64 *sorted = nullptr;
+    u32 *unused = nullptr;
+    radix_sort<u64>(ka.as<u64>(), kb.as<u64>(), nullptr, nullptr, np2, 2 * b, c.stream,
+                    &sorted, &unused);
+    pair_runs<<<grid_for(np2, 256), 256, 0, c.stream>>>(sorted, np2, b, dead,
+                                                       maxmult.as<unsigned int>(),
+                                                       rep.as<unsigned int>());
+    SP_CHECK_LAUNCH();
+    SP_HIP(hipStreamSynchronize(c.stream));
+  }
   DevBuf df((u64)n * NFEAT * sizeof(float));
   assemble_features<<<grid_for((u64)n * NFEAT, 256), 256, 0, c.stream>>>(
-      n, c.w_segptr.as<u64>(), partners.as<unsigned int>(), s0.as<double>(), df.as<float>());
+      n, c.w_segptr.as<u64>(), partners.as<unsigned int>(), maxmult.as<unsigned int>(),
+      rep.as<unsigned int>(), s0.as<double>(), df.as<float>());
   SP_CHECK_LAUNCH();
   SP_HIP(hipMemcpyAsync(f, df.p, (u64)n * NFEAT * sizeof(float), hipMemcpyDeviceToHost,
                         c.stream));

@@ -7,10 +7,13 @@ Adam run on the GPU in the library's HIP kernels (sp_node_features,
 sp_mlp_*); this module is the host-side glue:
 
   features(t)            64 features per node (sp_node_features)
-  WarmStartModel         feature standardisation + MLP 64 -> 256 -> 256 -> R
+  WarmStartModel         feature standardisation + MLP 64 -> 256 -> 256 -> 1
+                         scoring nodes by their row norm in the converged
+                         factor (rotation invariant); the start is the
+                         indicator columns of the R best-scored nodes
+  feature_start(...)     the non-learned start from the same features
   aligned_targets(...)   converged factors rotated (orthogonal Procrustes)
-                         onto one reference so that the regression target is
-                         well defined (U is only defined up to rotation)
+                         onto one reference (for comparing factors)
   hooi_trace(...)        TuckerHOOI's iteration (regularize, S3TTMc, leftsvd,
                          gemmtn) from a given start, keeping every factor, for
                          convergence measured by subspace angles
@@ -96,34 +99,111 @@ def aligned_targets(factors, ref=None):
 
 
 class WarmStartModel:
-    """Feature standardisation + MLP; targets are scaled by sqrt(n) (O(1) rows)."""
+    """Feature standardisation + MLP 64 -> 256 -> 256 -> 1 that scores nodes.
+
+    Target: the node's row norm in the factor the plain method converged to,
+    scaled so that the mean square over nodes is 1 (sqrt(n/R) * ||U*_i||).
+    The row norm is invariant to the rotation and column order of U*, so
+    factors of different tensors train one model without any alignment.
+    Start: the indicator columns of the R highest-scoring nodes (orthonormal),
+    the shape of the fixed points HOOI reaches on sparse planted tensors,
+    where the converged factor sits on a handful of nodes."""
 
     def __init__(self, rank, hidden=256, seed=0):
         self.rank = rank
-        self.mlp = MLP(NFEAT, hidden, rank, seed)
+        self.mlp = MLP(NFEAT, hidden, 1, seed)
         self.mu = None
         self.sd = None
-        self.ref = None
 
     def fit(self, feats, factors, steps=600, batch=4096, lr=1e-3, seed=0):
         """feats: list of (n x 64); factors: list of converged orthonormal U."""
-        targets, self.ref = aligned_targets(factors)
         X = np.concatenate(feats).astype(np.float64)
         self.mu = X.mean(axis=0)
         self.sd = X.std(axis=0) + 1e-6
         Xs = ((X - self.mu) / self.sd).astype(np.float32)
-        T = np.concatenate([np.sqrt(U.shape[0]) * U for U in targets]).astype(np.float32)
-        return self.mlp.train(Xs, T, steps, batch, lr, seed)
+        T = np.concatenate([np.sqrt(U.shape[0] / U.shape[1]) *
+                            np.sqrt((U * U).sum(axis=1, keepdims=True)) for U in factors])
+        return self.mlp.train(Xs, T.astype(np.float32), steps, batch, lr, seed)
 
-    def predict(self, feats):
+    def scores(self, feats):
         Xs = ((feats - self.mu) / self.sd).astype(np.float32)
-        return self.mlp.predict(Xs).astype(np.float64) / np.sqrt(feats.shape[0])
+        return self.mlp.predict(Xs)[:, 0].astype(np.float64)
+
+    def predict(self, feats, inds=None):
+        """Start for a tensor: with its index tuples, the hyperedge
+        neighbourhood of the best-scored node (pair_neighbourhood_start),
+        else the indicator columns of the top-scored nodes."""
+        sc = self.scores(feats)
+        if inds is not None:
+            return pair_neighbourhood_start(sc, inds, self.rank)
+        return top_nodes_start(sc, self.rank)
+
+
+def pair_neighbourhood_start(score, inds, rank):
+    """Indicator columns on the nodes around the best-scored node i: its
+    partner j sharing the most nonzeros with it, the nodes of the nonzeros
+    holding both (most shared first), then the best-scored remaining nodes.
+    A start on whole nonzeros keeps the first S3TTMc from vanishing (an
+    owner contributes only when all its leaf nodes carry weight)."""
+    score = np.asarray(score, dtype=np.float64)
+    n = len(score)
+    T = np.asarray(inds, dtype=np.int64) - 1
+    rank_of = np.empty(n, dtype=np.int64)
+    rank_of[np.lexsort((np.arange(n), -score))] = np.arange(n)
+    i = int(np.argmin(rank_of))
+    E = T[(T == i).any(axis=1)]
+    chosen = [i]
+    if len(E):
+        cnt = np.bincount(E.ravel(), minlength=n).astype(np.float64)
+        cnt[i] = -1.0
+        j = int(np.lexsort((rank_of, -cnt))[0])
+        if cnt[j] > 0:
+            B = E[(E == j).any(axis=1)]
+            cb = np.bincount(B.ravel(), minlength=n).astype(np.float64)
+            cb[[i, j]] = -1.0
+            nodes = np.nonzero(cb > 0)[0]
+            nodes = nodes[np.lexsort((rank_of[nodes], -cb[nodes]))]
+            chosen += [j] + [int(x) for x in nodes]
+    seen = set(chosen)
+    for x in np.argsort(rank_of):
+        if len(chosen) >= rank:
+            break
+        if int(x) not in seen:
+            chosen.append(int(x))
+            seen.add(int(x))
+    U = np.zeros((n, rank))
+    U[chosen[:rank], np.arange(rank)] = 1.0
+    return U
+
+
+def top_nodes_start(score, rank):
+    """Indicator columns of the `rank` highest-scoring nodes (ties: lower
+    node index first)."""
+    order = np.lexsort((np.arange(len(score)), -np.asarray(score)))
+    U = np.zeros((len(score), rank))
+    U[order[:rank], np.arange(rank)] = 1.0
+    return U
+
+
+def feature_score(feats):
+    """Non-learned node score from the same features: largest pair
+    multiplicity, then most repeated partners, then degree."""
+    F = np.asarray(feats, dtype=np.float64)
+    return F[:, 2] * 1e6 + F[:, 3] * 1e3 + F[:, 0]
+
+
+def feature_start(feats, rank, inds=None):
+    """Non-learned start: the same construction as the learned start, on
+    feature_score instead of the network's score."""
+    if inds is not None:
+        return pair_neighbourhood_start(feature_score(feats), inds, rank)
+    return top_nodes_start(feature_score(feats), rank)
 
 
 def spectral_start(feats, rank):
     """Non-learned start from the same features: the top-`rank` principal
     directions (left singular vectors) of the centred hypergraph sketch."""
-    X = np.asarray(feats, dtype=np.float64)[:, 2:]
+    X = np.asarray(feats, dtype=np.float64)[:, 4:]
     X = X - X.mean(axis=0)
     return np.linalg.svd(X, full_matrices=False)[0][:, :rank]
 

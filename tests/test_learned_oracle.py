@@ -78,6 +78,20 @@ def test_features_match_loop_restatement():
                 partners[a - 1].add(b)
                 partners[b - 1].add(a)
     assert np.allclose(F[:, 1], np.log1p([len(s) for s in partners]))
+    # pair multiplicity: occurrences of {a, b} over all position pairs
+    mult = {}
+    for row in t:
+        for a, b in itertools.combinations(row, 2):
+            if a != b:
+                mult[(a, b)] = mult.get((a, b), 0) + 1
+    maxmult, rep = np.zeros(n), np.zeros(n)
+    for (a, b), m in mult.items():
+        for x in (a, b):
+            maxmult[x - 1] = max(maxmult[x - 1], m)
+            rep[x - 1] += m >= 2
+    assert np.allclose(F[:, 2], np.log1p(maxmult))
+    assert np.allclose(F[:, 3], np.log1p(rep))
+    assert rep.max() >= 1   # the case exercises repeated pairs
     # propagation: s'[r] += v * sum of the other positions' vectors, per position
     s = mlp.node_gauss(n, mlp.SKETCH, 5)
     for _ in range(2):
@@ -88,4 +102,4 @@ def test_features_match_loop_restatement():
                 new[row[j]] += val * s[others].sum(axis=0)
         nrm = np.linalg.norm(new, axis=1, keepdims=True)
         s = np.where(nrm > 0, new / np.where(nrm > 0, nrm, 1), 0)
-    assert np.allclose(F[:, 2:], s, atol=1e-12)
+    assert np.allclose(F[:, 4:], s, atol=1e-12)
