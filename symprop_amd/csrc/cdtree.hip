@@ -370,10 +370,7 @@ void link_levels(Ctx &c, int l) {
   // CSR from level l (parents, KP) to level l+1 (children, KC)
   DevLevel &P = c.levels[l];
   DevLevel &C = c.levels[l + 1];
-  DevBuf fidx(((u64)c.dim + 2) * sizeof(u64)), cfidx(((u64)c.dim + 2) * sizeof(u64));
-  first_field_index<KP><<<grid_for((u64)c.dim + 2, BLK), BLK, 0, c.stream>>>(
-      P.keys.as<KP>(), P.size, l + 1, c.bits, c.dim, fidx.as<u64>());
-  SP_CHECK_LAUNCH();
+  DevBuf cfidx(((u64)c.dim + 2) * sizeof(u64));
   first_field_index<KC><<<grid_for((u64)c.dim + 2, BLK), BLK, 0, c.stream>>>(
       C.keys.as<KC>(), C.size, l + 2, c.bits, c.dim, cfidx.as<u64>());
   SP_CHECK_LAUNCH();
@@ -382,7 +379,18 @@ void link_levels(Ctx &c, int l) {
       C.keys.as<KC>(), C.size, c.bits, P.keys.as<KP>(), P.size, l + 1, cfidx.as<u64>(),
       P.level_ptr.as<u64>());
   SP_CHECK_LAUNCH();
-  // dep_list of level l+1 against level l
+  SP_HIP(hipStreamSynchronize(c.stream));
+}
+
+// dep_list of level l+1 against level l.
+template <class KC, class KP>
+void exclude_level(Ctx &c, int l) {
+  DevLevel &P = c.levels[l];
+  DevLevel &C = c.levels[l + 1];
+  DevBuf fidx(((u64)c.dim + 2) * sizeof(u64));
+  first_field_index<KP><<<grid_for((u64)c.dim + 2, BLK), BLK, 0, c.stream>>>(
+      P.keys.as<KP>(), P.size, l + 1, c.bits, c.dim, fidx.as<u64>());
+  SP_CHECK_LAUNCH();
   const int s = l + 2;
   C.dep_list.alloc(C.size * (u64)s * sizeof(u64));
   exclude_one<KC, KP><<<grid_for(C.size * (u64)s, BLK), BLK, 0, c.stream>>>(
@@ -435,6 +443,105 @@ void with_key(int words, F &&f) {
 
 }  // namespace
 
+namespace {
+
+// Input checks and ||X||^2 terms (spsymtensor.cpp:114-138): block b sums the
+// terms v^2 N!/prod f! of nonzeros [b*2048, (b+1)*2048) in a fixed order
+// (8 consecutive terms per thread, then a binary tree over the 256 threads)
+// into part[b]; a tuple with an index outside [1, dim] or out of order sets
+// bit 1 or 2 of *bad.  No atomics on the sum: the result does not depend on
+// scheduling.
+constexpr int kNormItems = 8;
+
+__global__ void __launch_bounds__(256)
+check_norm_partial(const u32 *__restrict__ inds, const double *__restrict__ vals, u64 nnz,
+                   int N, u32 dim, double *__restrict__ part, unsigned int *__restrict__ bad) {
+  __shared__ double red[256];
+  u64 f[20];
+  f[0] = 1;
+  for (int i = 1; i < 20; ++i) f[i] = f[i - 1] * (u64)i;
+  const u64 base = ((u64)blockIdx.x * 256 + threadIdx.x) * kNormItems;
+  double s = 0.0;
+  unsigned int flags = 0;
+  for (int k = 0; k < kNormItems; ++k) {
+    const u64 i = base + k;
+    if (i >= nnz) break;
+    const u32 *t = inds + i * N;
+    u64 denom = 1;
+    int run = 1;
+    u32 prev = t[0];
+    flags |= (prev < 1 || prev > dim) ? 1u : 0u;
+    for (int j = 1; j <= N; ++j) {
+      const u32 x = j < N ? t[j] : 0;
+      if (j < N) {
+        flags |= (x < 1 || x > dim) ? 1u : 0u;
+        flags |= (x < prev) ? 2u : 0u;
+      }
+      if (j < N && x == prev) {
+        ++run;
+      } else {
+        denom *= f[run];
+        run = 1;
+      }
+      prev = x;
+    }
+    const double v = vals[i];
+    s += v * v * (double)(f[N] / denom);
+  }
+  if (flags) atomicOr(bad, flags);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Sum of the block partials, same fixed pattern in one block.
+__global__ void __launch_bounds__(256)
+sum_norm_parts(const double *__restrict__ part, u64 n, double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (u64 i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace
+
+void input_check_norm(Ctx &c) {
+  const u64 nb = (c.nnz + 256 * kNormItems - 1) / (256 * kNormItems);
+  DevBuf part((nb + 1) * sizeof(double) + 256);
+  double *dpart = part.as<double>();
+  double *dsum = dpart + nb;
+  unsigned int *dbad = reinterpret_cast<unsigned int *>(dpart + nb + 1);
+  SP_HIP(hipMemsetAsync(dbad, 0, sizeof(unsigned int), c.stream));
+  check_norm_partial<<<(unsigned)nb, 256, 0, c.stream>>>(c.d_inds.as<u32>(),
+                                                         c.d_vals.as<double>(), c.nnz,
+                                                         (int)c.order, c.dim, dpart, dbad);
+  SP_CHECK_LAUNCH();
+  sum_norm_parts<<<1, 256, 0, c.stream>>>(dpart, nb, dsum);
+  SP_CHECK_LAUNCH();
+  double sum = 0.0;
+  unsigned int bad = 0;
+  SP_HIP(hipMemcpyAsync(&sum, dsum, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipMemcpyAsync(&bad, dbad, sizeof(unsigned int), hipMemcpyDeviceToHost, c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  if (bad & 1u) throw Error(SP_ERR_ARG, "index outside [1,dim]");
+  if (bad & 2u) throw Error(SP_ERR_ARG, "index tuple not sorted");
+  c.norm = std::sqrt(sum);
+}
+
+// The S3TTMc path needs the leaf level, the owners and the rem-sorted work
+// list only; those are built here.  The interior levels with their CSR
+// links (cdtree_ensure_levels) and the exclude-one parent lists
+// (cdtree_ensure_dep_list) are built on first use.
 void cdtree_build(Ctx &c) {
   const int N = (int)c.order;
   c.depth = N - 1;
@@ -447,27 +554,47 @@ void cdtree_build(Ctx &c) {
   c.key_words = key_words_for(N - 1, b);
   c.levels.clear();
   c.levels.resize(c.depth);
-  for (int l = 0; l < c.depth - 1; ++l) {
-    if (key_words_for(l + 1, b) == 1) build_interior_level<u64>(c, l);
-    else build_interior_level<u128>(c, l);
-  }
+  c.levels_built = false;
   DevBuf own_leaf;
   if (c.key_words == 1) build_leaf_level<u64>(c, own_leaf);
   else build_leaf_level<u128>(c, own_leaf);
-  for (int l = 0; l + 1 < c.depth; ++l) {
-    const int wp = key_words_for(l + 1, b), wc = key_words_for(l + 2, b);
-    if (wp == 1 && wc == 1) link_levels<u64, u64>(c, l);
-    else if (wp == 1) link_levels<u128, u64>(c, l);
-    else link_levels<u128, u128>(c, l);
-  }
   if (c.key_words == 1) build_worklist_t<u64>(c, own_leaf);
   else build_worklist_t<u128>(c, own_leaf);
   c.built = true;
   c.chunk_R = 0;  // work chunks depend on the tree
 }
 
+void cdtree_ensure_levels(Ctx &c) {
+  if (!c.built) throw Error(SP_ERR_STATE, "tree not built");
+  if (c.levels_built) return;
+  const int b = c.bits;
+  for (int l = 0; l < c.depth - 1; ++l) {
+    if (key_words_for(l + 1, b) == 1) build_interior_level<u64>(c, l);
+    else build_interior_level<u128>(c, l);
+  }
+  for (int l = 0; l + 1 < c.depth; ++l) {
+    const int wp = key_words_for(l + 1, b), wc = key_words_for(l + 2, b);
+    if (wp == 1 && wc == 1) link_levels<u64, u64>(c, l);
+    else if (wp == 1) link_levels<u128, u64>(c, l);
+    else link_levels<u128, u128>(c, l);
+  }
+  c.levels_built = true;
+}
+
+void cdtree_ensure_dep_list(Ctx &c, int l) {
+  if (l < 1 || l >= c.depth) return;
+  cdtree_ensure_levels(c);
+  if (c.levels[l].dep_list.p) return;
+  const int wp = key_words_for(l, c.bits), wc = key_words_for(l + 1, c.bits);
+  if (wp == 1 && wc == 1) exclude_level<u64, u64>(c, l - 1);
+  else if (wp == 1) exclude_level<u128, u64>(c, l - 1);
+  else exclude_level<u128, u128>(c, l - 1);
+}
+
 void cdtree_export_level(Ctx &c, int l, u32 *keys, u32 *level_idx, u64 *level_ptr,
                          u64 *dep_list) {
+  if (l < c.depth - 1) cdtree_ensure_levels(c);
+  if (dep_list) cdtree_ensure_dep_list(c, l);
   DevLevel &lv = c.levels[l];
   const int s = l + 1;
   const u64 L = lv.size;

@@ -548,6 +548,7 @@ void node_features(Ctx &c, int rounds, u64 seed, float *f) {
   DevBuf partners((u64)n * sizeof(unsigned int));
   SP_HIP(hipMemsetAsync(partners.p, 0, (u64)n * sizeof(unsigned int), c.stream));
   if (c.depth >= 2) {
+    if (c.depth > 2) cdtree_ensure_levels(c);
     // level 1 holds pairs: 2 * bits(dim) <= 64, so its keys are u64
     const DevLevel &l1 = c.levels[1];
     partner_counts<<<grid_for(l1.size, 256), 256, 0, c.stream>>>(

@@ -75,13 +75,6 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
     need(order >= 3 && order <= 16, SP_ERR_UNSUPPORTED, "order must be in [3,16]");
     need(dim >= 1, SP_ERR_ARG, "dim must be >= 1");
     need(nnz >= 1, SP_ERR_ARG, "nnz must be >= 1");
-    for (uint64_t i = 0; i < nnz; ++i) {
-      const uint32_t *t = inds + i * order;
-      for (uint32_t j = 0; j < order; ++j) {
-        need(t[j] >= 1 && t[j] <= dim, SP_ERR_ARG, "index outside [1,dim]");
-        need(j == 0 || t[j - 1] <= t[j], SP_ERR_ARG, "index tuple not sorted");
-      }
-    }
     int ndev = 0;
     SP_HIP(hipGetDeviceCount(&ndev));
     need(ndev > 0, SP_ERR_DEVICE, "no HIP device");
@@ -91,7 +84,6 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
       c.order = order;
       c.dim = dim;
       c.nnz = nnz;
-      c.norm = host_norm(order, nnz, inds, vals);
       if (const char *e = std::getenv("SP_TRD_SPIN"))
         c.trd_spin_limit = (unsigned)std::max(1L, std::strtol(e, nullptr, 10));
       if (const char *e = std::getenv("SP_EIG"))
@@ -104,12 +96,17 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
       SP_HIP(hipStreamCreate(&c.stream));
       c.d_inds.alloc(nnz * order * sizeof(uint32_t));
       c.d_vals.alloc(nnz * sizeof(double));
-      SP_HIP(hipMemcpy(c.d_inds.p, inds, nnz * order * sizeof(uint32_t),
-                       hipMemcpyHostToDevice));
-      SP_HIP(hipMemcpy(c.d_vals.p, vals, nnz * sizeof(double), hipMemcpyHostToDevice));
+      SP_HIP(hipMemcpyAsync(c.d_inds.p, inds, nnz * order * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, c.stream));
+      SP_HIP(hipMemcpyAsync(c.d_vals.p, vals, nnz * sizeof(double), hipMemcpyHostToDevice,
+                            c.stream));
+      // index range and order checks and the norm, on the device
+      input_check_norm(c);
     } catch (...) {
-      if (c.stream) (void)hipStreamDestroy(c.stream);
+      if (c.stream) (void)hipStreamSynchronize(c.stream);
+      hipStream_t s = c.stream;
       delete h;
+      if (s) (void)hipStreamDestroy(s);
       throw;
     }
     *out = h;
@@ -168,6 +165,7 @@ int sp_cdtree_level_size(sp_ctx *ctx, uint32_t level, uint64_t *size) {
     need(ctx && size, SP_ERR_ARG, "null argument");
     need(ctx->c.built, SP_ERR_STATE, "tree not built");
     need(level < (uint32_t)ctx->c.depth, SP_ERR_ARG, "level out of range");
+    if ((int)level < ctx->c.depth - 1) cdtree_ensure_levels(ctx->c);
     *size = ctx->c.levels[level].size;
   });
 }

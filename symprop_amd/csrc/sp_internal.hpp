@@ -140,6 +140,7 @@ struct Ctx {
   DevBuf d_vals;  // f64 nnz
 
   bool built = false;
+  bool levels_built = false;  // interior levels and their CSR links
   std::vector<DevLevel> levels;
   DevBuf own_rem;   // u32 [owners], leaf order
   DevBuf own_val;   // f64 [owners], leaf order
@@ -220,7 +221,11 @@ struct Ctx {
 };
 
 // construction (cdtree.hip)
+// Checks the uploaded tuples (indices in [1, dim], sorted) and sets c.norm.
+void input_check_norm(Ctx &c);
 void cdtree_build(Ctx &c);
+void cdtree_ensure_levels(Ctx &c);
+void cdtree_ensure_dep_list(Ctx &c, int l);
 void cdtree_export_level(Ctx &c, int l, u32 *keys, u32 *level_idx,
                          u64 *level_ptr, u64 *dep_list);
 void cdtree_export_owners(Ctx &c, u32 *rem, double *val);

@@ -59,6 +59,26 @@ def test_construction_contact5(contact5):
     assert t.norm() == g["norm"]
 
 
+def test_ctx_create_rejects_bad_tuples():
+    import ctypes as C
+    L = sp.lib()
+    h = C.c_void_p()
+    v = np.ones(3)
+    say("ctx create: unsorted and out-of-range tuples")
+    t = np.array([[1, 2, 3], [2, 1, 3], [1, 1, 1]], np.uint32)
+    rc = L.sp_ctx_create(3, 5, 3, t.ctypes.data_as(sp.u32p), v.ctypes.data_as(sp.f64p),
+                         C.byref(h))
+    assert rc == sp.SP_ERR_ARG and b"sorted" in L.sp_last_error() and not h.value
+    for bad in ([1, 2, 9], [0, 1, 2]):
+        t = np.array([[1, 2, 3], bad, [1, 1, 1]], np.uint32)
+        rc = L.sp_ctx_create(3, 5, 3, t.ctypes.data_as(sp.u32p), v.ctypes.data_as(sp.f64p),
+                             C.byref(h))
+        assert rc == sp.SP_ERR_ARG and b"outside" in L.sp_last_error() and not h.value
+    # the norm: a fixed-order sum of v^2 N!/prod f! (here 6 + 3 + 1 = 10 for values 1)
+    t = sp.SpSymtensor(5, 3, np.array([[1, 2, 3], [1, 1, 2], [4, 4, 4]], np.uint32), v)
+    assert t.norm() == np.sqrt(10.0)
+
+
 def test_construction_small_golden():
     z = np.load(os.path.join(GOLDEN, "small_trees.npz"))
     for ci in range(5):
