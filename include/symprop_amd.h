@@ -285,6 +285,17 @@ int sp_sync(sp_ctx *ctx);
 int sp_load_tns(const char *path, uint32_t *order, uint32_t *dim,
                 uint64_t *nnz, uint32_t *inds, double *vals);
 
+/* The same reader with a binary tuple cache (no counterpart in the
+ * reference).  When `cache` names a file that an earlier call wrote for
+ * this source (same size and modification time), the sorted tuples and
+ * values are read from it; otherwise the text is parsed and the cache file
+ * written (replaced atomically).  Text files are parsed by several host
+ * threads; a file whose tokens are not all plain numbers is read by the
+ * serial stream reader instead.  The first call of the two-call protocol
+ * keeps the tuples in host memory for the second. */
+int sp_load_tns_cached(const char *path, const char *cache, uint32_t *order,
+                       uint32_t *dim, uint64_t *nnz, uint32_t *inds, double *vals);
+
 /* Seeded synthetic tensors (unique sorted tuples, lexicographically sorted):
  *   kind 0: uniform indices over [1,dim], values 1.0
  *   kind 1: bounded Zipf(s) node popularity over [1,dim], values U(0,1)

@@ -92,6 +92,8 @@ SIGNATURES = {
     "sp_timing_reset": (ctypes.c_int, [vp]),
     "sp_sync": (ctypes.c_int, [vp]),
     "sp_load_tns": (ctypes.c_int, [ctypes.c_char_p, u32p, u32p, u64p, u32p, f64p]),
+    "sp_load_tns_cached": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, u32p, u32p, u64p,
+                                          u32p, f64p]),
     "sp_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.c_uint64, ctypes.c_double, ctypes.c_uint32,
                                    ctypes.c_uint64, u32p, f64p, u64p]),
@@ -141,14 +143,21 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
-def load_tns(path):
+def load_tns(path, cache=None):
+    """(order, dim, inds n x order, vals) of a .tns file; with `cache`, through
+    the binary tuple cache (sp_load_tns_cached)."""
     o, d, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
-    _check(lib().sp_load_tns(path.encode(), ctypes.byref(o), ctypes.byref(d),
-                             ctypes.byref(n), None, None))
+
+    def call(i, v):
+        if cache is None:
+            return lib().sp_load_tns(path.encode(), ctypes.byref(o), ctypes.byref(d),
+                                     ctypes.byref(n), i, v)
+        return lib().sp_load_tns_cached(path.encode(), cache.encode(), ctypes.byref(o),
+                                        ctypes.byref(d), ctypes.byref(n), i, v)
+    _check(call(None, None))
     inds = np.zeros(n.value * o.value, dtype=np.uint32)
     vals = np.zeros(n.value)
-    _check(lib().sp_load_tns(path.encode(), ctypes.byref(o), ctypes.byref(d),
-                             ctypes.byref(n), _p(inds, u32p), _p(vals, f64p)))
+    _check(call(_p(inds, u32p), _p(vals, f64p)))
     return o.value, d.value, inds.reshape(n.value, o.value), vals
 
 

@@ -1,4 +1,4 @@
-// Input side: tensor norm, .tns reader and seeded synthetic generators.
+// Input side: seeded synthetic generators (the .tns reader is in tns_reader.cpp).
 #include "sp_internal.hpp"
 
 #include <algorithm>
@@ -68,38 +68,6 @@ double host_norm(u32 order, u64 nnz, const u32 *inds, const double *vals) {
 using namespace spamd;
 
 extern "C" {
-
-// src/utils/loader.cpp:8-36
-int sp_load_tns(const char *path, uint32_t *order, uint32_t *dim, uint64_t *nnz,
-                uint32_t *inds, double *vals) {
-  if (!path || !order || !dim || !nnz) return SP_ERR_ARG;
-  std::ifstream f(path);
-  if (!f.is_open()) return SP_ERR_IO;
-  u32 o = 0, d = 0;
-  f >> o >> d;
-  if (o == 0 || o > 64) return SP_ERR_IO;
-  std::vector<u32> t;
-  u64 cnt = 0;
-  while (true) {
-    t.clear();
-    for (u32 i = 0; i < o; ++i) {
-      u32 v;
-      f >> v;
-      t.push_back(v);
-    }
-    double v;
-    f >> v;
-    if (!f.good()) break;
-    std::sort(t.begin(), t.end());
-    if (inds) std::copy(t.begin(), t.end(), inds + cnt * o);
-    if (vals) vals[cnt] = v;
-    ++cnt;
-  }
-  *order = o;
-  *dim = d;
-  *nnz = cnt;
-  return SP_OK;
-}
 
 int sp_generate(int kind, uint32_t order, uint32_t dim, uint64_t nnz, double param,
                 uint32_t communities, uint64_t seed, uint32_t *inds, double *vals,

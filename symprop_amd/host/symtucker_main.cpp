@@ -5,9 +5,10 @@
 // "Option X is not defined" message, exit status 1 for -h and for unknown
 // flags, the "Insertion time" and per-algorithm "Running"/"Time" lines, the
 // output sinks ("" -> discarded, "stdout", "stderr", else a file path), and
-// -a -1 (default) running HOOI, HOQRI and AltHOOI in turn.  Extension flag:
+// -a -1 (default) running HOOI, HOQRI and AltHOOI in turn.  Extension flags:
 // -p <0|1|2> selects FP64 / FP32 / BF16 S3TTMc iterations (the reduced forms
-// finish with FP64 iterations).
+// finish with FP64 iterations); -c <file> reads the tensor through a binary
+// tuple cache (written on first use, reused while the .tns file is unchanged).
 //
 // Design: options are rows of a table (flag, value parser), algorithms are
 // rows of a second table (id, printed name, member function), so adding a
@@ -37,7 +38,8 @@ using symprop::SymTuckerDecomp;
 struct Invocation {
   Config config;
   std::string tensor_path;
-  std::string sink;  // where the decomposition is written
+  std::string cache_path;  // binary tuple cache (optional)
+  std::string sink;        // where the decomposition is written
   int alg = -1;      // -1: all algorithms
 };
 
@@ -50,7 +52,7 @@ struct OptionRow {
 
 // Numeric conversions follow the reference: -i goes through a double, the
 // integer flags through int.
-const std::array<OptionRow, 10> kOptions{{
+const std::array<OptionRow, 11> kOptions{{
     {"-f", [](Invocation &v, const char *s) { v.tensor_path = s; }},
     {"-i", [](Invocation &v, const char *s) {
        v.config.max_iter = static_cast<symprop::index_t>(std::stod(s));
@@ -67,6 +69,7 @@ const std::array<OptionRow, 10> kOptions{{
     {"-o", [](Invocation &v, const char *s) { v.sink = s; }},
     {"-init", [](Invocation &v, const char *s) { v.config.init = s; }},
     {"-p", [](Invocation &v, const char *s) { v.config.precision = std::stoi(s); }},
+    {"-c", [](Invocation &v, const char *s) { v.cache_path = s; }},
 }};
 
 struct AlgorithmRow {
@@ -133,17 +136,23 @@ SymTuckerDecomp run_one(const SpSymtensor &t, const AlgorithmRow &a, const Confi
   return result;
 }
 
-// The tensor file through the library's loader (two passes: shape, then data).
-void load_tensor(const std::string &path, symprop::order_t &order, symprop::dim_t &dim,
-                 std::vector<symprop::dim_t> &inds, std::vector<symprop::real_t> &vals) {
+// The tensor file through the library's loader (two calls: shape, then data).
+void load_tensor(const std::string &path, const std::string &cache, symprop::order_t &order,
+                 symprop::dim_t &dim, std::vector<symprop::dim_t> &inds,
+                 std::vector<symprop::real_t> &vals) {
+  const char *cp = cache.empty() ? nullptr : cache.c_str();
+  auto call = [&](uint64_t &nnz, uint32_t *i, double *v) {
+    return cp ? sp_load_tns_cached(path.c_str(), cp, &order, &dim, &nnz, i, v)
+              : sp_load_tns(path.c_str(), &order, &dim, &nnz, i, v);
+  };
   uint64_t nnz = 0;
-  if (sp_load_tns(path.c_str(), &order, &dim, &nnz, nullptr, nullptr) != SP_OK) {
+  if (call(nnz, nullptr, nullptr) != SP_OK) {
     std::cerr << "Error opening file: " << path << std::endl;
     throw std::runtime_error("Error opening file in SymtensLoader");
   }
   inds.resize(static_cast<size_t>(nnz) * order);
   vals.resize(nnz);
-  if (sp_load_tns(path.c_str(), &order, &dim, &nnz, inds.data(), vals.data()) != SP_OK)
+  if (call(nnz, inds.data(), vals.data()) != SP_OK)
     throw std::runtime_error(std::string("symtucker: ") + sp_last_error());
 }
 
@@ -157,7 +166,7 @@ int main(int argc, char **argv) {
   symprop::dim_t dim = 0;
   std::vector<symprop::dim_t> inds;
   std::vector<symprop::real_t> vals;
-  load_tensor(inv.tensor_path, order, dim, inds, vals);
+  load_tensor(inv.tensor_path, inv.cache_path, order, dim, inds, vals);
 
   const auto t0 = std::chrono::high_resolution_clock::now();
   const SpSymtensor tensor(dim, order, inds, vals);

@@ -54,6 +54,69 @@ def test_loader_matches_oracle_loader(tmp_path):
     assert a[3].tolist() == b[3].tolist() == [0.25, -2.0]
 
 
+def _write_tns(path, order, dim, tuples, vals, sep=" ", eol="\n", trailing=True):
+    lines = ["%d %d" % (order, dim)]
+    for t, v in zip(tuples, vals):
+        lines.append(sep.join(str(int(x)) for x in t) + sep + repr(float(v)))
+    text = eol.join(lines) + (eol if trailing else "")
+    path.write_text(text)
+
+
+@pytest.mark.parametrize("sep,eol,trailing", [(" ", "\n", True), ("\t", "\r\n", True),
+                                              ("  ", "\n", False)])
+def test_parallel_loader_matches_stream_reader(tmp_path, sep, eol, trailing):
+    # several MiB, so the text is split over several parser threads; tuples
+    # unsorted in the file, values with exponents and signs
+    rng = np.random.default_rng(5)
+    n = 120_000
+    t = rng.integers(1, 5000, size=(n, 5))
+    v = rng.standard_normal(n) * 10.0 ** rng.integers(-5, 5, size=n)
+    p = tmp_path / "big.tns"
+    _write_tns(p, 5, 5000, t, v, sep, eol, trailing)
+    a = sp.load_tns(str(p))
+    b = oracle.load_tns(str(p))
+    assert a[0] == b[0] == 5 and a[1] == b[1] == 5000
+    # without a trailing newline the stream drops the last tuple
+    assert len(a[3]) == n - (0 if trailing else 1)
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+def test_loader_falls_back_to_stream_semantics(tmp_path):
+    # tokens the fast path does not take: the serial stream reader decides
+    for body in ("1 2 3 0.5\n3 2 1 1.5abc\n4 4 4 2\n", "1 2 3 +0.5\n2 2 2 1\n",
+                 "1 2 3 nan\n", "1 2 99999999999 1\n", "1 2 3 1\n1 2 x 1\n"):
+        p = tmp_path / "odd.tns"
+        p.write_text("3 9\n" + body)
+        a = sp.load_tns(str(p))
+        b = oracle.load_tns(str(p))
+        assert a[2].tolist() == b[2].tolist(), body
+        assert a[3].tolist() == b[3].tolist(), body
+
+
+def test_loader_binary_cache(tmp_path):
+    rng = np.random.default_rng(6)
+    t = rng.integers(1, 300, size=(5000, 4))
+    v = rng.random(5000)
+    p = tmp_path / "c.tns"
+    c = tmp_path / "c.spbin"
+    _write_tns(p, 4, 300, t, v)
+    a = sp.load_tns(str(p))
+    b = sp.load_tns(str(p), cache=str(c))       # parses and writes the cache
+    assert c.exists()
+    c_mtime = os.stat(c).st_mtime_ns
+    d = sp.load_tns(str(p), cache=str(c))       # reads the cache
+    assert os.stat(c).st_mtime_ns == c_mtime
+    for x in (b, d):
+        assert x[0] == a[0] and x[1] == a[1]
+        assert np.array_equal(x[2], a[2]) and np.array_equal(x[3], a[3])
+    # a changed source invalidates the cache
+    _write_tns(p, 4, 300, t[:100], v[:100])
+    os.utime(p, ns=(os.stat(p).st_atime_ns, os.stat(p).st_mtime_ns + 10**9))
+    e = sp.load_tns(str(p), cache=str(c))
+    assert len(e[3]) == 100
+    assert np.array_equal(e[2], oracle.load_tns(str(p))[2])
+
+
 def test_loader_missing_file():
     with pytest.raises(sp.SpError) as e:
         sp.load_tns("/nonexistent/x.tns")
