@@ -597,15 +597,20 @@ def test_hoevd_factor_matches_oracle_eigenspace(n, cnt, eig, monkeypatch):
     assert np.array_equal(t.factor(), U)
 
 
-@pytest.mark.skip(reason="hung inside rocsolver_dsyevdx (index range 297..300 of this "
-                         "300 x 300 matrix); that solver was removed from the HOEVD path "
-                         "and this input is kept off the GPU")
 def test_hoevd_init_in_tucker_run():
+    # This 300 x 300 HOEVD matrix once hung rocSOLVER's index-range solver
+    # dsyevdx (il = 297, iu = 300), which no path calls any more: the top-4
+    # pairs come from the library's reduction, host stage and back-transform
+    # (the host stage on this matrix is also checked on the CPU,
+    # tests/test_tridiag_host.py::test_hoevd_gram_300_through_the_host_stage).
     inds, vals = sp.generate(sp.GEN_ZIPF, 4, 300, 6000, 1.1, 1, seed=8)
     R = 4
-    say("tucker with hoevd init")
+    say("tucker with hoevd init, n = 300")
     t = sp.SpSymtensor(300, 4, inds, vals)
-    U0, _ = t.hoevd_init(R)
+    U0, w = t.hoevd_init(R)
+    Uo, wo = oracle.hoevd_factor(oracle.hoevd_gram(4, 300, inds, vals), R)
+    assert np.allclose(w, wo, rtol=1e-12)
+    assert np.allclose(np.abs(np.sum(U0 * Uo, axis=0)), 1.0, atol=1e-9)
     a = t.tucker(alg=1, rank=R, max_iter=3, rel_tol=0.0, init="hoevd")
     b = t.tucker(alg=1, rank=R, max_iter=3, rel_tol=0.0, init="given", U0=U0)
     assert np.array_equal(a["errors"], b["errors"])

@@ -149,3 +149,60 @@ def test_cli_runs_and_writes_reference_format(tmp_path):
     assert len(lines) == 1 + 4 + 1 + 245
     assert all(l.endswith(",") for l in lines[1:5] + lines[6:])
     assert len(lines[1].split(",")) == 4 ** 4 + 1
+
+
+def _parse_cli_output(path, R, n):
+    lines = path.read_text().splitlines()
+    assert lines[0] == "Mode-1 matricized C:"
+    assert lines[R + 1] == "Factor U:"
+    assert len(lines) == R + 2 + n
+    assert all(l.endswith(",") for l in lines[1:R + 1] + lines[R + 2:])
+    core = [np.array([float(x) for x in l.split(",")[:-1]]) for l in lines[1:R + 1]]
+    U = np.array([[float(x) for x in l.split(",")[:-1]] for l in lines[R + 2:]])
+    return core, U
+
+
+@pytest.mark.parametrize("alg", [0, 2])
+def test_cli_output_values_match_oracle(tmp_path, contact5, alg):
+    """The written decomposition (tucker.cpp:47-61) value by value against the
+    oracle's run of the same algorithm: HOOI (alg 0) writes the unpacked core
+    R x R^(N-1), AltHOOI (alg 2) the packed R x P core into the same R^N
+    array (quirk A2); both print core row i as the R^(N-1) entries starting at
+    flat index i*R (quirk A1), so consecutive rows overlap.  U is defined up
+    to the sign of each column (distinct singular values here), so factor
+    columns are compared after fixing their signs, and core entries -- which
+    carry products of those signs -- by absolute value.  Values are printed
+    with 6 significant digits."""
+    import subprocess
+    o, d, inds, vals = contact5
+    R, iters = 4, 3
+    out = tmp_path / "out.txt"
+    say("cli values, alg", alg)
+    p = subprocess.run([sp.CLI_PATH, "-f", os.path.join(GOLDEN, "contact-5.tns"), "-r", str(R),
+                        "-i", str(iters), "-a", str(alg), "-d", "0.1", "-o", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    core_rows, U = _parse_cli_output(out, R, d)
+    ofn = tucker.hooi if alg == 0 else tucker.althooi
+    Uo, eo, core_o = ofn(oracle.Tree(o, d, inds, vals), R, iters, rel_tol=1e-6, delta=0.1,
+                         seed=0)
+    # per-iteration errors printed with 12 (HOOI) / 10 (AltHOOI) digits
+    errs = [float(l.split(" error ")[1]) for l in p.stdout.splitlines()
+            if l.startswith("Iteration:")]
+    assert len(errs) == len(eo)
+    assert np.allclose(errs, eo, rtol=0, atol=1e-9)
+    # the flat core array as the reference fills it
+    flat = np.zeros(R ** o)
+    flat[:core_o.size] = core_o.ravel()
+    width = R ** (o - 1)
+    scale = np.abs(flat).max()
+    for i, row in enumerate(core_rows):
+        assert len(row) == width
+        ref = flat[i * R:i * R + width]
+        assert np.allclose(np.abs(row), np.abs(ref), rtol=2e-5, atol=2e-6 * scale), i
+        if i > 0:   # overlapping windows print the same entries
+            assert np.array_equal(row[:width - R], core_rows[i - 1][R:])
+    # factor columns up to sign
+    sgn = np.sign((U * Uo).sum(axis=0))
+    assert np.all(sgn != 0)
+    assert np.allclose(U * sgn, Uo, rtol=0, atol=2e-6)

@@ -96,3 +96,25 @@ def test_tridiag_top_matches_eigh(harness, name, d, e, R):
 def test_tridiag_top_rejects_bad_sizes(harness):
     p = subprocess.run([harness], input="3 4\n1 2 3\n0.5 0.5\n", capture_output=True, text=True)
     assert p.returncode != 0
+
+
+def test_hoevd_gram_300_through_the_host_stage(harness):
+    """The 300 x 300 HOEVD Gram of the order-4 Zipf tensor (n = 300, 6,000
+    tuples, seed 8) whose top-4 eigenpairs once hung a rocSOLVER index-range
+    solver: its tridiagonal form (scipy's Householder reduction) through the
+    library's host stage, against numpy's eigh of the Gram itself."""
+    import oracle
+    import symprop_amd as sp
+    inds, vals = sp.generate(sp.GEN_ZIPF, 4, 300, 6000, 1.1, 1, seed=8)
+    A = oracle.hoevd_gram(4, 300, inds, vals)
+    H, Q = sl.hessenberg(A, calc_q=True)
+    d, e = np.diag(H).copy(), np.diag(H, 1).copy()
+    R = 4
+    w, Z = run(harness, d, e, R)
+    lam, V = np.linalg.eigh(A)
+    scale = lam.max()
+    assert np.abs(w - lam[::-1][:R]).max() <= 1e-13 * scale
+    X = Q @ Z.T   # eigenvectors of A
+    assert np.linalg.norm(A @ X - X * w, axis=0).max() <= 1e-12 * scale
+    top = V[:, ::-1][:, :R]
+    assert np.allclose(np.abs(np.sum(X * top, axis=0)), 1.0, atol=1e-9)
