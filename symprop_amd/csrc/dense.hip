@@ -559,6 +559,20 @@ void complete_basis(Ctx &c, u32 good) {
 
 rocblas_handle blas_handle(Ctx &c) { return handle_for(c); }
 
+void dense_atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *out) {
+  atb(c, A, R, B, P, n, out, "atb");
+}
+
+void dense_ymt(Ctx &c, const double *Y, u64 n, u64 P, const double *M, u32 R, double *out) {
+  y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(Y, n, P, M, R, out);
+  SP_CHECK_LAUNCH();
+}
+
+void dense_small_eig(const std::vector<double> &A, int n, std::vector<double> &w,
+                     std::vector<double> &V) {
+  jacobi_eig(A, n, w, V);
+}
+
 void dense_gemmtn(Ctx &c) {
   c.d_core.reserve((u64)c.R * c.P * sizeof(double));
   atb(c, c.d_U.as<double>(), c.R, c.d_Y.as<double>(), c.P, c.dim,

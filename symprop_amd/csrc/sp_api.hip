@@ -91,6 +91,8 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
                        : std::strcmp(e, "syevj") == 0              ? 1
                        : std::strcmp(e, "tridiag-rocsolver") == 0  ? 3
                                                                    : 2;
+      if (const char *e = std::getenv("SP_HOEVD"))
+        c.hoevd_method = std::strcmp(e, "dense") == 0 ? 1 : std::strcmp(e, "krylov") == 0 ? 2 : 0;
       SP_HIP(hipGetDevice(&c.device));
       noalloc_setup(c.device);
       SP_HIP(hipStreamCreate(&c.stream));

@@ -185,6 +185,8 @@ struct Ctx {
                        // beyond its LDS limit) + host top-R tridiagonal solve +
                        // rocSOLVER ormtr; 3: as 2 with rocSOLVER sytrd always;
                        // 0: rocSOLVER syevd (divide and conquer); 1: syevj
+  int hoevd_method = 0;  // HOEVD solver: 0 (default) dense for dim <= 4096, block
+                         // Krylov on the implicit operator above; 1: dense; 2: Krylov
   DevBuf ws_sym;   // symmetric matrix uploaded by sp_syevd_top
   DevBuf ws_trd;   // tridiagonal reduction: broadcast vectors, barrier words
   DevBuf ws_trd_v; // tridiagonal reduction: reflector tails (P x P)
@@ -266,7 +268,15 @@ void dense_qr(Ctx &c);
 void dense_regularize(Ctx &c, double delta);
 double dense_core_norm(Ctx &c);
 void dense_factor_snapshot(Ctx &c);
+// out = A^T B (A: n x R, B: n x P, row-major; out R x P), fixed summation order.
+void dense_atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *out);
+// out = Y M^T (Y: n x P, M: R x P, out: n x R), queued on the context's stream.
+void dense_ymt(Ctx &c, const double *Y, u64 n, u64 P, const double *M, u32 R, double *out);
 double dense_subspace_sin(Ctx &c);
+// Host: eigen-decomposition of a small symmetric matrix (row-major n x n),
+// eigenvalues descending in w, eigenvectors in the columns of V (row-major).
+void dense_small_eig(const std::vector<double> &A, int n, std::vector<double> &w,
+                     std::vector<double> &V);
 
 // HOEVD initial factor (hoevd.hip)
 void hoevd_gram(Ctx &c, double *A);
