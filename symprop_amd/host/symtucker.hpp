@@ -55,6 +55,7 @@ public:
   void print(std::ostream &os) const;
 
   // Extensions: iteration record of the run that produced this result.
+  order_t order() const { return _order; }
   const std::vector<real_t> &factor() const { return _factor; }
   const std::vector<real_t> &core() const { return _core; }
   const std::vector<real_t> &errors() const { return _errors; }
@@ -77,7 +78,7 @@ private:
 
   dim_t _dim;
   dim_t _rank;
-  [[maybe_unused]] order_t _order;
+  order_t _order;
 
   std::vector<real_t> _core;
   std::vector<real_t> _factor;
