@@ -145,17 +145,20 @@ __global__ void transpose_cols(const double *__restrict__ Z, u64 n, u32 R,
 
 // ---- A V without forming A (the top-R solver for large dim) ----------------
 //
-// A V = sum_S mult(S) s_S (s_S^T V), taken 16 columns of V at a time:
+// A leaf S with a single distinct owner (rem r, value v) adds mult(S) v^2 to
+// A[r][r] and nothing else, so A = D + sum over the other leaves, with the
+// diagonal D computed once.  For the rest, A V is taken 16 columns of V at a
+// time:
 //   stage 1  C[S, :] = mult(S) * sum over the distinct owners (rem, v) of leaf S
 //            of v * V[rem-1, :]           (owners in leaf order, level_ptr)
-//   stage 2  Y[r, :] = sum over the distinct owners (leaf S, v) of row r
-//            of v * C[S, :]               (owners in rem order, the work list)
-// Both stages are segmented sums of gathered 16-wide rows.  Segments with at
-// most kLightOwners owners (most leaves) take 16 lanes each; longer ones are
-// cut into pieces of at most kPieceOwners owners, one wavefront per piece
-// (four 16-lane groups over every fourth owner), whose partial rows
-// seg_finish adds in piece order.  Every sum has a fixed order: the result
-// is deterministic.
+//   stage 2  Y[r, :] = D[r] V[r, :] + sum over the distinct owners (leaf S, v)
+//            of row r of v * C[S, :]      (owners in rem order: the work list
+//                                          without the single-owner leaves)
+// Both stages are segmented sums of gathered 16-wide rows.  Leaves with at
+// most kLightOwners owners take 16 lanes each; longer segments are cut into
+// pieces of at most kPieceOwners owners, one wavefront per piece (four
+// 16-lane groups over every fourth owner), whose partial rows seg_finish adds
+// in piece order.  Every sum has a fixed order: the result is deterministic.
 constexpr int kCols = 16;
 constexpr u64 kLightOwners = 32;
 constexpr u64 kPieceOwners = 256;
@@ -172,18 +175,103 @@ __global__ void leaf_mult(const K *__restrict__ keys, u64 L, int m, int b,
   mult[s] = leaf_multiplicity(t, m);
 }
 
-// stage 1 for the light leaves: 16 lanes per leaf, C row-major L x 16
-__global__ void leaf_rows_light(const u64 *__restrict__ ptr, u64 L,
+// Leaf kinds: 0 one distinct owner (diagonal only), 1 several, at most
+// kLightOwners owner entries, 2 more entries (cut into pieces).  light and
+// heavy get 0/1 flags for the compaction scans, with a 0 after the last leaf.
+__global__ void leaf_kinds(const u64 *__restrict__ ptr, u64 L, const u32 *__restrict__ own_rem,
+                           unsigned char *__restrict__ kind, u64 *__restrict__ light,
+                           u64 *__restrict__ heavy) {
+  const u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > L) return;
+  if (s == L) {
+    light[s] = heavy[s] = 0;
+    return;
+  }
+  const u64 p0 = ptr[s], p1 = ptr[s + 1];
+  int k = 2;
+  if (p1 - p0 <= kLightOwners) {
+    k = 0;
+    for (u64 p = p0 + 1; p < p1; ++p)
+      if (own_rem[p] != own_rem[p - 1]) k = 1;
+  }
+  kind[s] = (unsigned char)k;
+  light[s] = k == 1;
+  heavy[s] = k == 2;
+}
+
+// list[pos[s]] = s for the flagged s < L
+__global__ void compact_ids(const u64 *__restrict__ flag, const u64 *__restrict__ pos, u64 L,
+                            u32 *__restrict__ list) {
+  const u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < L && flag[s]) list[pos[s]] = (u32)s;
+}
+
+// the owner range of each listed leaf
+__global__ void leaf_ranges(const u32 *__restrict__ list, u64 nl, const u64 *__restrict__ ptr,
+                            u64 *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  out[2 * t] = ptr[list[t]];
+  out[2 * t + 1] = ptr[list[t] + 1];
+}
+
+// work-list entries whose leaf has several owners (flag, with a 0 after the last)
+__global__ void entry_flags(const u32 *__restrict__ w_leaf, u64 E,
+                            const unsigned char *__restrict__ kind, u64 *__restrict__ flag) {
+  const u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < E) flag[q] = kind[w_leaf[q]] != 0;
+  else if (q == E) flag[q] = 0;
+}
+
+__global__ void compact_entries(const u64 *__restrict__ flag, const u64 *__restrict__ pos, u64 E,
+                                const u32 *__restrict__ w_leaf, const double *__restrict__ w_val,
+                                u32 *__restrict__ leaf2, double *__restrict__ val2) {
+  const u64 q = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < E && flag[q]) {
+    leaf2[pos[q]] = w_leaf[q];
+    val2[pos[q]] = w_val[q];
+  }
+}
+
+// seg2[r] = pos[segptr[r]], r <= n: the row pointer of the compacted work list
+__global__ void remap_rows(const u64 *__restrict__ segptr, const u64 *__restrict__ pos, u64 n,
+                           u64 *__restrict__ seg2) {
+  const u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r <= n) seg2[r] = pos[segptr[r]];
+}
+
+// D[r] = sum over row r's distinct single-owner leaves of mult * v^2, one
+// wavefront per row (lanes over the row's entries, a fixed-order wave sum)
+__global__ void __launch_bounds__(64)
+diag_rows(const u64 *__restrict__ segptr, const u32 *__restrict__ w_leaf,
+          const double *__restrict__ w_val, const unsigned char *__restrict__ kind,
+          const double *__restrict__ mult, double *__restrict__ D) {
+  const u64 r = blockIdx.x;
+  const u64 q0 = segptr[r], q1 = segptr[r + 1];
+  double acc = 0.0;
+  for (u64 q = q0 + threadIdx.x; q < q1; q += 64) {
+    const u32 leaf = w_leaf[q];
+    if (kind[leaf] != 0 || (q > q0 && w_leaf[q - 1] == leaf)) continue;
+    const double v = w_val[q];
+    acc += mult[leaf] * (v * v);
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) D[r] = acc;
+}
+
+// stage 1 for the light leaves with several owners: 16 lanes per listed leaf,
+// C row-major L x 16 (rows of the other leaves are not touched)
+__global__ void leaf_rows_light(const u32 *__restrict__ list, u64 nl, const u64 *__restrict__ ptr,
                                 const u32 *__restrict__ own_rem,
                                 const double *__restrict__ own_val,
                                 const double *__restrict__ mult, const double *__restrict__ V,
                                 int ldv, int col0, double *__restrict__ C) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  const u64 s = t / kCols;
+  const u64 i = t / kCols;
   const int j = (int)(t % kCols);
-  if (s >= L) return;
+  if (i >= nl) return;
+  const u64 s = list[i];
   const u64 p0 = ptr[s], p1 = ptr[s + 1];
-  if (p1 - p0 > kLightOwners) return;  // cut into pieces
   double acc = 0.0;
   for (u64 p = p0; p < p1; ++p) {
     const u32 rem = own_rem[p];
@@ -223,9 +311,11 @@ struct SegOut {
   u64 p0, p1;  // its pieces
 };
 
-// out[row, ocol0 + j] = (scale ? scale[row] : 1) * sum of its pieces, in order
+// out[row, ocol0 + j] = (scale ? scale[row] : 1) * (sum of its pieces, in
+// order) [+ diag[row] * X[row, ocol0 + j] when diag is given; X: ld ldo]
 __global__ void seg_finish(const SegOut *__restrict__ segs, u64 nseg,
                            const double *__restrict__ part, const double *__restrict__ scale,
+                           const double *__restrict__ diag, const double *__restrict__ X,
                            double *__restrict__ out, int ldo, int ocol0) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const u64 s = t / kCols;
@@ -235,7 +325,9 @@ __global__ void seg_finish(const SegOut *__restrict__ segs, u64 nseg,
   double acc = 0.0;
   for (u64 p = so.p0; p < so.p1; ++p) acc += part[p * kCols + j];
   if (scale) acc *= scale[so.row];
-  out[so.row * ldo + ocol0 + j] = acc;
+  const u64 o = so.row * ldo + ocol0 + j;
+  if (diag) acc += diag[so.row] * X[o];
+  out[o] = acc;
 }
 
 // Q[:, j0 : j0 + w] = W (W: n x w, Q: n x ldq, row-major)
@@ -304,16 +396,27 @@ void hoevd_gram(Ctx &c, double *A) {
 namespace {
 
 // The implicit HOEVD operator: A V for n x w row-major V (w a multiple of
-// 16), from the leaf level's owner lists and the rem-sorted work list.
+// 16), from the leaf level's owner lists and the rem-sorted work list.  The
+// set-up (leaf kinds, the diagonal, the compacted lists) runs on the device;
+// only the pieces of the leaves and rows longer than a piece are cut on the
+// host, from their owner ranges.
 class HoevdOperator {
  public:
   explicit HoevdOperator(Ctx &c) : c_(c), n_(c.dim) {
     const DevLevel &lv = c.levels[c.depth - 1];
     L_ = lv.size;
+    const u64 E = c.h_segptr.size() == n_ + 1 ? c.h_segptr[n_] : 0;
+    if (c.h_segptr.size() != n_ + 1) throw Error(SP_ERR_STATE, "work list not built");
     const int m = (int)c.order - 1;
+    const u64 *ptr = lv.level_ptr.as<u64>();
     mult_.alloc(std::max<u64>(L_, 1) * sizeof(double));
     C_.alloc(std::max<u64>(L_, 1) * kCols * sizeof(double));
-    if (L_ > 0) {
+    diag_.alloc(std::max<u64>(n_, 1) * sizeof(double));
+    std::vector<SegPiece> lp, rp;
+    std::vector<SegOut> ls, rs;
+    {
+      DevBuf kind(L_ + 1), light((L_ + 1) * sizeof(u64)), heavy((L_ + 1) * sizeof(u64));
+      DevBuf lpos((L_ + 1) * sizeof(u64)), hpos((L_ + 1) * sizeof(u64));
       if (c.key_words == 1)
         leaf_mult<<<grid_for(L_, 256), 256, 0, c.stream>>>(lv.keys.as<u64>(), L_, m, c.bits,
                                                           mult_.as<double>());
@@ -321,22 +424,71 @@ class HoevdOperator {
         leaf_mult<<<grid_for(L_, 256), 256, 0, c.stream>>>(lv.keys.as<u128>(), L_, m, c.bits,
                                                           mult_.as<double>());
       SP_CHECK_LAUNCH();
+      leaf_kinds<<<grid_for(L_ + 1, 256), 256, 0, c.stream>>>(
+          ptr, L_, c.own_rem.as<u32>(), kind.as<unsigned char>(), light.as<u64>(),
+          heavy.as<u64>());
+      SP_CHECK_LAUNCH();
+      exclusive_scan_u64(light.as<u64>(), lpos.as<u64>(), L_ + 1, c.stream);
+      exclusive_scan_u64(heavy.as<u64>(), hpos.as<u64>(), L_ + 1, c.stream);
+      u64 nh = 0;
+      SP_HIP(hipMemcpyAsync(&n_light_, lpos.as<u64>() + L_, sizeof(u64), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipMemcpyAsync(&nh, hpos.as<u64>() + L_, sizeof(u64), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipStreamSynchronize(c.stream));
+      light_.alloc(std::max<u64>(n_light_, 1) * sizeof(u32));
+      DevBuf hl(std::max<u64>(nh, 1) * sizeof(u32)), hr(std::max<u64>(nh, 1) * 2 * sizeof(u64));
+      compact_ids<<<grid_for(L_, 256), 256, 0, c.stream>>>(light.as<u64>(), lpos.as<u64>(), L_,
+                                                          light_.as<u32>());
+      SP_CHECK_LAUNCH();
+      compact_ids<<<grid_for(L_, 256), 256, 0, c.stream>>>(heavy.as<u64>(), hpos.as<u64>(), L_,
+                                                          hl.as<u32>());
+      SP_CHECK_LAUNCH();
+      // pieces of the heavy leaves
+      if (nh > 0) {
+        leaf_ranges<<<grid_for(nh, 256), 256, 0, c.stream>>>(hl.as<u32>(), nh, ptr,
+                                                            hr.as<u64>());
+        SP_CHECK_LAUNCH();
+        std::vector<u32> ids(nh);
+        std::vector<u64> rg(2 * nh);
+        SP_HIP(hipMemcpyAsync(ids.data(), hl.p, nh * sizeof(u32), hipMemcpyDeviceToHost,
+                              c.stream));
+        SP_HIP(hipMemcpyAsync(rg.data(), hr.p, 2 * nh * sizeof(u64), hipMemcpyDeviceToHost,
+                              c.stream));
+        SP_HIP(hipStreamSynchronize(c.stream));
+        for (u64 i = 0; i < nh; ++i) cut(rg[2 * i], rg[2 * i + 1], ids[i], lp, ls);
+      }
+      // the diagonal, and the work list without the single-owner leaves
+      diag_rows<<<(unsigned)std::max<u64>(n_, 1), 64, 0, c.stream>>>(
+          c.w_segptr.as<u64>(), c.w_leaf.as<u32>(), c.w_val.as<double>(),
+          kind.as<unsigned char>(), mult_.as<double>(), diag_.as<double>());
+      SP_CHECK_LAUNCH();
+      DevBuf ef((E + 1) * sizeof(u64)), epos((E + 1) * sizeof(u64));
+      DevBuf seg2((n_ + 1) * sizeof(u64));
+      entry_flags<<<grid_for(E + 1, 256), 256, 0, c.stream>>>(
+          c.w_leaf.as<u32>(), E, kind.as<unsigned char>(), ef.as<u64>());
+      SP_CHECK_LAUNCH();
+      exclusive_scan_u64(ef.as<u64>(), epos.as<u64>(), E + 1, c.stream);
+      u64 E2 = 0;
+      SP_HIP(hipMemcpyAsync(&E2, epos.as<u64>() + E, sizeof(u64), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipStreamSynchronize(c.stream));
+      leaf2_.alloc(std::max<u64>(E2, 1) * sizeof(u32));
+      val2_.alloc(std::max<u64>(E2, 1) * sizeof(double));
+      compact_entries<<<grid_for(E, 256), 256, 0, c.stream>>>(
+          ef.as<u64>(), epos.as<u64>(), E, c.w_leaf.as<u32>(), c.w_val.as<double>(),
+          leaf2_.as<u32>(), val2_.as<double>());
+      SP_CHECK_LAUNCH();
+      remap_rows<<<grid_for(n_ + 1, 256), 256, 0, c.stream>>>(c.w_segptr.as<u64>(),
+                                                             epos.as<u64>(), n_, seg2.as<u64>());
+      SP_CHECK_LAUNCH();
+      std::vector<u64> hs(n_ + 1);
+      SP_HIP(hipMemcpyAsync(hs.data(), seg2.p, (n_ + 1) * sizeof(u64), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipStreamSynchronize(c.stream));
+      // pieces of every row (an empty row gets an empty piece range)
+      for (u64 r = 0; r < n_; ++r) cut(hs[r], hs[r + 1], r, rp, rs);
     }
-    // pieces of the leaves with more than kLightOwners owners
-    std::vector<u64> ptr(L_ + 1);
-    SP_HIP(hipMemcpyAsync(ptr.data(), lv.level_ptr.p, (L_ + 1) * sizeof(u64),
-                          hipMemcpyDeviceToHost, c.stream));
-    SP_HIP(hipStreamSynchronize(c.stream));
-    std::vector<SegPiece> lp;
-    std::vector<SegOut> ls;
-    for (u64 s = 0; s < L_; ++s)
-      if (ptr[s + 1] - ptr[s] > kLightOwners) cut(ptr[s], ptr[s + 1], s, lp, ls);
-    // pieces of every row (an empty row gets an empty piece range)
-    std::vector<SegPiece> rp;
-    std::vector<SegOut> rs;
-    const std::vector<u64> &seg = c.h_segptr;
-    if (seg.size() != n_ + 1) throw Error(SP_ERR_STATE, "work list not built");
-    for (u64 r = 0; r < n_; ++r) cut(seg[r], seg[r + 1], r, rp, rs);
     n_leaf_pieces_ = lp.size();
     n_leaf_segs_ = ls.size();
     n_row_pieces_ = rp.size();
@@ -352,10 +504,10 @@ class HoevdOperator {
     ScopedTimer t(c_, "hoevd_apply");
     const DevLevel &lv = c_.levels[c_.depth - 1];
     for (int col0 = 0; col0 < w; col0 += kCols) {
-      if (L_ > 0) {
-        leaf_rows_light<<<grid_for(L_ * kCols, 256), 256, 0, c_.stream>>>(
-            lv.level_ptr.as<u64>(), L_, c_.own_rem.as<u32>(), c_.own_val.as<double>(),
-            mult_.as<double>(), V, w, col0, C_.as<double>());
+      if (n_light_ > 0) {
+        leaf_rows_light<<<grid_for(n_light_ * kCols, 256), 256, 0, c_.stream>>>(
+            light_.as<u32>(), n_light_, lv.level_ptr.as<u64>(), c_.own_rem.as<u32>(),
+            c_.own_val.as<double>(), mult_.as<double>(), V, w, col0, C_.as<double>());
         SP_CHECK_LAUNCH();
       }
       if (n_leaf_pieces_ > 0) {
@@ -365,17 +517,18 @@ class HoevdOperator {
         SP_CHECK_LAUNCH();
         seg_finish<<<grid_for(n_leaf_segs_ * kCols, 256), 256, 0, c_.stream>>>(
             leaf_segs_.as<SegOut>(), n_leaf_segs_, part_.as<double>(), mult_.as<double>(),
-            C_.as<double>(), kCols, 0);
+            nullptr, nullptr, C_.as<double>(), kCols, 0);
         SP_CHECK_LAUNCH();
       }
       if (n_row_pieces_ > 0) {
         seg_pieces<<<(unsigned)n_row_pieces_, 64, 0, c_.stream>>>(
-            row_pieces_.as<SegPiece>(), c_.w_leaf.as<u32>(), 0u, c_.w_val.as<double>(),
+            row_pieces_.as<SegPiece>(), leaf2_.as<u32>(), 0u, val2_.as<double>(),
             C_.as<double>(), kCols, 0, part_.as<double>());
         SP_CHECK_LAUNCH();
       }
       seg_finish<<<grid_for(n_ * kCols, 256), 256, 0, c_.stream>>>(
-          row_segs_.as<SegOut>(), n_, part_.as<double>(), nullptr, Y, w, col0);
+          row_segs_.as<SegOut>(), n_, part_.as<double>(), nullptr, diag_.as<double>(), V, Y, w,
+          col0);
       SP_CHECK_LAUNCH();
     }
   }
@@ -398,9 +551,10 @@ class HoevdOperator {
   }
 
   Ctx &c_;
-  u64 n_, L_ = 0;
+  u64 n_, L_ = 0, n_light_ = 0;
   u64 n_leaf_pieces_ = 0, n_leaf_segs_ = 0, n_row_pieces_ = 0;
-  DevBuf mult_, C_, part_, leaf_pieces_, leaf_segs_, row_pieces_, row_segs_;
+  DevBuf mult_, C_, diag_, light_, leaf2_, val2_, part_;
+  DevBuf leaf_pieces_, leaf_segs_, row_pieces_, row_segs_;
 };
 
 // Block width and basis size of the Krylov solve; false where it does not
