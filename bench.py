@@ -467,6 +467,23 @@ def fit_study(t, alg, R, delta, long_iters, sub_tol=1e-7):
                           "forms": conv}}
 
 
+def hoevd_timing(t, R):
+    """The HOEVD initial factor (top-R eigenvectors of X_(1) X_(1)^T,
+    spsymtensor.cpp:363-382) on the workload tensor, with the solver the
+    library picks for its dim, timed after a first call (which also sets up
+    the operator's buffers)."""
+    t0 = time.perf_counter()
+    t.hoevd_init(R)
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, w = t.hoevd_init(R)
+    sec = time.perf_counter() - t0
+    return {"solver": ("block Krylov on the implicit X(1) X(1)^T" if t.dim > 4096 else
+                       "dense X(1) X(1)^T and the top-R eigen-step"),
+            "first_s": first, "seconds": sec,
+            "top_eigenvalues": [float(x) for x in w[::-1]]}
+
+
 def planted_basis(n, K):
     E = np.zeros((n, K))
     b = n // K
@@ -836,6 +853,8 @@ def main():
         print(json.dumps(out), flush=True)
         return
     out["dense_y_side"] = dense_y_side(t, N, n, R, w["delta"])
+    out["hoevd_init"] = hoevd_timing(t, R)
+    log("HOEVD init %.3f s (%s)" % (out["hoevd_init"]["seconds"], out["hoevd_init"]["solver"]))
 
     alg = w.get("alg", 0)
     aname = ("hooi", "hoqri")[alg]
