@@ -220,8 +220,12 @@ int sp_subspace_sin(sp_ctx *ctx, double *out);
  * sp_hoevd_init: eigenvectors of the `R` largest eigenvalues of A, ascending
  * (as matrix::syevd's last R columns, src/dense/matrixops.cpp:77-100); the
  * result becomes the context's factor and is copied to U (dim x R, row-major)
- * and the eigenvalues to eigvals (R) when those are not NULL.  Dense
- * eigensolver: dim^2 must stay below 2^31. */
+ * and the eigenvalues to eigvals (R) when those are not NULL.  Up to
+ * dim = 4096 A is formed and solved densely (dim^2 below 2^31); above, block
+ * Lanczos runs on the implicit A (no dim x dim matrix; R up to about dim/64,
+ * SP_ERR_UNSUPPORTED beyond).  SP_HOEVD=dense|krylov, read at sp_ctx_create,
+ * forces one of the two.  SP_ERR_NUMERIC if the Krylov solve does not reach
+ * residuals of 1e-12 times the largest eigenvalue. */
 int sp_hoevd_gram(sp_ctx *ctx, double *A);
 int sp_hoevd_init(sp_ctx *ctx, uint32_t R, double *U, double *eigvals);
 
