@@ -222,8 +222,9 @@ int sp_subspace_sin(sp_ctx *ctx, double *out);
  * result becomes the context's factor and is copied to U (dim x R, row-major)
  * and the eigenvalues to eigvals (R) when those are not NULL.  Up to
  * dim = 4096 A is formed and solved densely (dim^2 below 2^31); above, block
- * Lanczos runs on the implicit A (no dim x dim matrix; R up to about dim/64,
- * SP_ERR_UNSUPPORTED beyond).  SP_HOEVD=dense|krylov, read at sp_ctx_create,
+ * Lanczos runs on the implicit A (no dim x dim matrix) when its basis of four
+ * blocks of 16*ceil(R/16) columns fits, i.e. about R <= min(dim/4, 512,
+ * 1.25e8/dim), and the dense path is used otherwise.  SP_HOEVD=dense|krylov, read at sp_ctx_create,
  * forces one of the two.  SP_ERR_NUMERIC if the Krylov solve does not reach
  * residuals of 1e-12 times the largest eigenvalue. */
 int sp_hoevd_gram(sp_ctx *ctx, double *A);
