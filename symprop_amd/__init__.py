@@ -13,6 +13,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsymprop_amd.so")
 CLI_PATH = os.path.join(_HERE, "symtucker")
+# the S3TTMc micro-drivers (examples/symtens_ttmc.cpp, examples/s3ttmctc.cpp)
+TTMC_PATH = os.path.join(_HERE, "symtens_ttmc")
+TTMCTC_PATH = os.path.join(_HERE, "s3ttmctc")
 
 SP_OK = 0
 SP_ERR_ARG = 1

@@ -2,8 +2,9 @@
 
 Usage: python -m symprop_amd.build   (or __graft_entry__.build()).
 Sources are compiled in parallel with hipcc into symprop_amd/_build/ and
-linked in-tree as symprop_amd/libsymprop_amd.so; the symtucker CLI is linked
-as symprop_amd/symtucker.
+linked in-tree as symprop_amd/libsymprop_amd.so; the command-line drivers
+(symtucker, and the S3TTMc micro-drivers symtens_ttmc and s3ttmctc) are linked
+next to it.
 """
 import os
 import subprocess
@@ -16,6 +17,10 @@ HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsymprop_amd.so")
 CLI = os.path.join(HERE, "symtucker")
+# driver executables: (main source under host/, executable in the package)
+DRIVERS = [("symtucker_main.cpp", CLI),
+           ("symtens_ttmc_main.cpp", os.path.join(HERE, "symtens_ttmc")),
+           ("s3ttmctc_main.cpp", os.path.join(HERE, "s3ttmctc"))]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
@@ -34,7 +39,7 @@ def _sources():
         if not os.path.isdir(d):
             continue
         for f in sorted(os.listdir(d)):
-            if f.endswith((".hip", ".cpp")) and f != "symtucker_main.cpp":
+            if f.endswith((".hip", ".cpp")) and f not in {m for m, _ in DRIVERS}:
                 srcs.append(os.path.join(d, f))
     return srcs
 
@@ -69,11 +74,10 @@ def build(verbose=False):
                    ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
                     "-Wl,-rpath," + os.path.join(ROCM, "lib")], check=True)
     os.replace(tmp, LIB)
-    main = os.path.join(HOST, "symtucker_main.cpp")
-    if os.path.exists(main):
-        subprocess.run([HIPCC] + COMMON + [main, "-o", CLI, "-L" + HERE, "-lsymprop_amd",
-                        "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")],
-                       check=True)
+    for main, exe in DRIVERS:
+        subprocess.run([HIPCC] + COMMON + [os.path.join(HOST, main), "-o", exe,
+                        "-L" + HERE, "-lsymprop_amd", "-Wl,-rpath,$ORIGIN",
+                        "-Wl,-rpath," + os.path.join(ROCM, "lib")], check=True)
     return LIB
 
 

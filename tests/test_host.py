@@ -173,3 +173,21 @@ def test_ctx_create_rejects_bad_arguments_before_touching_a_gpu():
     assert rc == sp.SP_ERR_UNSUPPORTED
     rc = L.sp_ctx_create(3, 5, 1, None, None, C.byref(h))
     assert rc == sp.SP_ERR_ARG
+
+
+@pytest.mark.parametrize("exe", ["TTMC_PATH", "TTMCTC_PATH"])
+def test_micro_drivers_usage_and_missing_file(tmp_path, exe):
+    # examples/symtens_ttmc.cpp:12-15 and :27-29 (s3ttmctc.cpp the same): the
+    # usage line and the open error, both with exit status 1, before any
+    # device work
+    import subprocess
+    path = getattr(sp, exe)
+    if not os.path.exists(path):
+        pytest.skip("drivers not built")
+    p = subprocess.run([path], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1
+    assert p.stderr == "Usage: %s <input_file> <R>\n" % path
+    p = subprocess.run([path, str(tmp_path / "missing.tns"), "4"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1
+    assert p.stderr == "Error opening file: %s\n" % (tmp_path / "missing.tns")
