@@ -240,25 +240,6 @@ __global__ void remap_rows(const u64 *__restrict__ segptr, const u64 *__restrict
   if (r <= n) seg2[r] = pos[segptr[r]];
 }
 
-// D[r] = sum over row r's distinct single-owner leaves of mult * v^2, one
-// wavefront per row (lanes over the row's entries, a fixed-order wave sum)
-__global__ void __launch_bounds__(64)
-diag_rows(const u64 *__restrict__ segptr, const u32 *__restrict__ w_leaf,
-          const double *__restrict__ w_val, const unsigned char *__restrict__ kind,
-          const double *__restrict__ mult, double *__restrict__ D) {
-  const u64 r = blockIdx.x;
-  const u64 q0 = segptr[r], q1 = segptr[r + 1];
-  double acc = 0.0;
-  for (u64 q = q0 + threadIdx.x; q < q1; q += 64) {
-    const u32 leaf = w_leaf[q];
-    if (kind[leaf] != 0 || (q > q0 && w_leaf[q - 1] == leaf)) continue;
-    const double v = w_val[q];
-    acc += mult[leaf] * (v * v);
-  }
-  acc = wave_sum(acc);
-  if (threadIdx.x == 0) D[r] = acc;
-}
-
 // stage 1 for the light leaves with several owners: 16 lanes per listed leaf,
 // C row-major L x 16 (rows of the other leaves are not touched)
 __global__ void leaf_rows_light(const u32 *__restrict__ list, u64 nl, const u64 *__restrict__ ptr,
@@ -328,6 +309,39 @@ __global__ void seg_finish(const SegOut *__restrict__ segs, u64 nseg,
   const u64 o = so.row * ldo + ocol0 + j;
   if (diag) acc += diag[so.row] * X[o];
   out[o] = acc;
+}
+
+// D[r] = sum over row r's distinct single-owner leaves of mult * v^2.  The
+// rem-sorted work list is cut into pieces of at most kDiagPieceOwners
+// entries (never across rows); one wavefront sums a piece (lanes over its
+// entries, a fixed-order wave sum) and diag_finish adds a row's pieces in
+// order.
+constexpr u64 kDiagPieceOwners = 4096;
+
+__global__ void __launch_bounds__(64)
+diag_pieces(const SegPiece *__restrict__ pieces, const u32 *__restrict__ w_leaf,
+            const double *__restrict__ w_val, const unsigned char *__restrict__ kind,
+            const double *__restrict__ mult, double *__restrict__ part) {
+  const SegPiece pc = pieces[blockIdx.x];
+  double acc = 0.0;
+  for (u64 q = pc.o0 + threadIdx.x; q < pc.o1; q += 64) {
+    const u32 leaf = w_leaf[q];
+    if (kind[leaf] != 0 || (q > pc.first && w_leaf[q - 1] == leaf)) continue;
+    const double v = w_val[q];
+    acc += mult[leaf] * (v * v);
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void diag_finish(const SegOut *__restrict__ segs, u64 n,
+                            const double *__restrict__ part, double *__restrict__ D) {
+  const u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const SegOut so = segs[r];
+  double acc = 0.0;
+  for (u64 p = so.p0; p < so.p1; ++p) acc += part[p];
+  D[so.row] = acc;
 }
 
 // Q[:, j0 : j0 + w] = W (W: n x w, Q: n x ldq, row-major)
@@ -458,11 +472,28 @@ class HoevdOperator {
         SP_HIP(hipStreamSynchronize(c.stream));
         for (u64 i = 0; i < nh; ++i) cut(rg[2 * i], rg[2 * i + 1], ids[i], lp, ls);
       }
-      // the diagonal, and the work list without the single-owner leaves
-      diag_rows<<<(unsigned)std::max<u64>(n_, 1), 64, 0, c.stream>>>(
-          c.w_segptr.as<u64>(), c.w_leaf.as<u32>(), c.w_val.as<double>(),
-          kind.as<unsigned char>(), mult_.as<double>(), diag_.as<double>());
-      SP_CHECK_LAUNCH();
+      // the diagonal
+      {
+        std::vector<SegPiece> dp;
+        std::vector<SegOut> ds;
+        for (u64 r = 0; r < n_; ++r)
+          cut(c.h_segptr[r], c.h_segptr[r + 1], r, dp, ds, kDiagPieceOwners);
+        DevBuf dpd, dsd, dpart(std::max<size_t>(dp.size(), 1) * sizeof(double));
+        upload(dp, dpd);
+        upload(ds, dsd);
+        if (!dp.empty()) {
+          diag_pieces<<<(unsigned)dp.size(), 64, 0, c.stream>>>(
+              dpd.as<SegPiece>(), c.w_leaf.as<u32>(), c.w_val.as<double>(),
+              kind.as<unsigned char>(), mult_.as<double>(), dpart.as<double>());
+          SP_CHECK_LAUNCH();
+        }
+        diag_finish<<<grid_for(n_, 256), 256, 0, c.stream>>>(dsd.as<SegOut>(), n_,
+                                                             dpart.as<double>(),
+                                                             diag_.as<double>());
+        SP_CHECK_LAUNCH();
+        SP_HIP(hipStreamSynchronize(c.stream));  // dpd, dsd, dpart are freed here
+      }
+      // the work list without the single-owner leaves
       DevBuf ef((E + 1) * sizeof(u64)), epos((E + 1) * sizeof(u64));
       DevBuf seg2((n_ + 1) * sizeof(u64));
       entry_flags<<<grid_for(E + 1, 256), 256, 0, c.stream>>>(
@@ -535,10 +566,9 @@ class HoevdOperator {
 
  private:
   static void cut(u64 o0, u64 o1, u64 row, std::vector<SegPiece> &pieces,
-                  std::vector<SegOut> &segs) {
+                  std::vector<SegOut> &segs, u64 len = kPieceOwners) {
     const u64 p0 = pieces.size();
-    for (u64 q = o0; q < o1; q += kPieceOwners)
-      pieces.push_back({q, std::min(o1, q + kPieceOwners), o0});
+    for (u64 q = o0; q < o1; q += len) pieces.push_back({q, std::min(o1, q + len), o0});
     segs.push_back({row, p0, (u64)pieces.size()});
   }
   template <class T>
