@@ -376,7 +376,8 @@ def dense_y_side(t, N, n, R, delta, iters=3):
     t.sync()
     ops = {}
     for name in ("s3ttmc_mfma64", "s3ttmc_generic", "reduce_split_rows", "gram", "syevd",
-                 "y_times_v", "gemmtn", "gram_u", "qr"):
+                 "syevj", "sytrd", "tridiag_top", "ormtr", "y_times_v", "gemmtn", "gram_u",
+                 "qr"):
         ms, cnt = t.timing_get(name)
         if cnt:
             ops[name] = ms / iters
