@@ -15,11 +15,17 @@
 // with s_S the sparse vector of the leaf's distinct owners.  No permutation
 // list, no key overflow for n^(N-1) >= 2^64.
 //
-// Row r of A is produced by one wavefront from the owners whose rem is r
-// (the rem-sorted work list, leaves ascending), scattering into an LDS tile of
-// A[r, c0 : c0+TILE].  Contributions are added in ascending leaf order, each
-// as mult * (v_r * v_c), so A[r][c] and A[c][r] are the same floating-point
-// sum and A is exactly symmetric; the result is deterministic.
+// Two solvers, chosen by dim (hoevd_init):
+//  - dense (dim <= 4096): row r of A is produced by one wavefront from the
+//    owners whose rem is r (the rem-sorted work list, leaves ascending),
+//    scattering into an LDS tile of A[r, c0 : c0+TILE].  Contributions are
+//    added in ascending leaf order, each as mult * (v_r * v_c), so A[r][c]
+//    and A[c][r] are the same floating-point sum and A is exactly symmetric;
+//    then the top-R eigen-step on A.
+//  - block Krylov (larger dim): A is never formed; HoevdOperator applies it
+//    to blocks of vectors from the same owner lists and hoevd_krylov runs
+//    block Lanczos with full reorthogonalisation on it.
+// Both are deterministic.
 #include "sp_internal.hpp"
 #include "wave.hpp"
 
