@@ -339,13 +339,16 @@ void s3ttmc_prepare(Ctx &c, u32 R) {
   c.fast_kernel = fast_kernel_for(m, R, &c.fast_H, &c.fast_bo, &c.fast_kernel_f32,
                                   &c.fast_kernel_bf16);
   if (c.fast_kernel) {
-    std::vector<u32> fptr, fterm;
-    make_fold_table(R, m, c.fast_H, fptr, fterm);
+    std::vector<u32> fptr, fterm, ftab;
+    make_fold_table(R, m, c.fast_H, fptr, fterm, ftab, c.fold_len);
     c.d_fold_ptr.alloc(fptr.size() * sizeof(u32));
     c.d_fold_term.alloc(std::max<size_t>(1, fterm.size()) * sizeof(u32));
+    c.d_fold_tab.alloc(ftab.size() * sizeof(u32));
     SP_HIP(hipMemcpy(c.d_fold_ptr.p, fptr.data(), fptr.size() * sizeof(u32),
                      hipMemcpyHostToDevice));
     SP_HIP(hipMemcpy(c.d_fold_term.p, fterm.data(), fterm.size() * sizeof(u32),
+                     hipMemcpyHostToDevice));
+    SP_HIP(hipMemcpy(c.d_fold_tab.p, ftab.data(), ftab.size() * sizeof(u32),
                      hipMemcpyHostToDevice));
   }
 
@@ -424,21 +427,21 @@ void s3ttmc_launch(Ctx &c, int precision, const double *U, float *U32, double *Y
     launch_fast(c.fast_kernel, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
                 pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), c.w_coef.as<double>(), U,
                 nullptr, nullptr, c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
-                c.d_fact.as<double>(), Y, partial);
+                c.d_fold_tab.as<u32>(), c.fold_len, c.d_fact.as<double>(), Y, partial);
     SP_CHECK_LAUNCH();
   } else if (precision == SP_PREC_FP32 && c.fast_kernel_f32) {
     ScopedTimer t(c, "s3ttmc_mfma_f32");
     launch_fast(c.fast_kernel_f32, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
                 pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
                 c.w_coef32.as<float>(), U32, c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
-                c.d_fact.as<double>(), Y, partial);
+                c.d_fold_tab.as<u32>(), c.fold_len, c.d_fact.as<double>(), Y, partial);
     SP_CHECK_LAUNCH();
   } else if (precision == SP_PREC_BF16 && c.fast_kernel_bf16) {
     ScopedTimer t(c, "s3ttmc_mfma_bf16");
     launch_fast(c.fast_kernel_bf16, nchunks, c.stream, pl.d_chunks.as<Chunk>(),
                 pl.d_batches.as<Batch>(), c.w_tuple.as<u32>(), nullptr, nullptr,
                 c.w_coef32.as<float>(), U32, c.d_fold_ptr.as<u32>(), c.d_fold_term.as<u32>(),
-                c.d_fact.as<double>(), Y, partial);
+                c.d_fold_tab.as<u32>(), c.fold_len, c.d_fact.as<double>(), Y, partial);
     SP_CHECK_LAUNCH();
   } else {
     GenArgs a{};

@@ -213,7 +213,9 @@ struct Ctx {
   void *fast_kernel_bf16 = nullptr;
   int fast_H = 0;
   int fast_bo = 64;  // owners per batch of the fast kernels
-  DevBuf d_fold_ptr, d_fold_term;
+  DevBuf d_fold_ptr, d_fold_term;  // fold lists of the MFMA kernels (CSR)
+  DevBuf d_fold_tab;               // the same, thread-major
+  int fold_len = 0;
 
   // profiling
   bool profile = false;
@@ -252,12 +254,14 @@ u64 s3ttmc_partial_slots(const Ctx &c, int precision);
 // Allocation-free form of the launch above (noalloc.hip).
 void noalloc_setup(int device);
 void s3ttmc_noalloc(Ctx &c, const double *U, u32 R, double *Y, int precision);
-void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term);
+void make_fold_table(u32 R, int M, int H, std::vector<u32> &ptr, std::vector<u32> &term,
+                     std::vector<u32> &tab, int &len);
 void *fast_kernel_for(int M, u32 R, int *H, int *bo, void **kf32, void **kbf16);
 void launch_fast(void *kernel, unsigned grid, hipStream_t s, const Chunk *chunks,
                  const Batch *batches, const u32 *tup, const double *coef, const double *U,
                  const float *coef32, const float *U32, const u32 *fold_ptr,
-                 const u32 *fold_term, const double *fact, double *Y, double *partial);
+                 const u32 *fold_term, const u32 *fold_tab, int fold_len, const double *fact,
+                 double *Y, double *partial);
 
 // dense Y-side ops (dense.hip)
 void dense_gemmtn(Ctx &c);
