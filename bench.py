@@ -376,16 +376,18 @@ def dense_y_side(t, N, n, R, delta, iters=3):
     t.sync()
     ops = {}
     for name in ("s3ttmc_mfma64", "s3ttmc_generic", "reduce_split_rows", "gram", "syevd",
-                 "syevj", "sytrd", "tridiag_top", "ormtr", "y_times_v", "gemmtn", "gram_u",
-                 "qr"):
+                 "syevj", "eig_warm", "sytrd", "tridiag_top", "ormtr", "y_times_v", "gemmtn",
+                 "gram_u", "qr"):
         ms, cnt = t.timing_get(name)
         if cnt:
             ops[name] = ms / iters
+    warm = {"attempts": t.timing_get("eig_warm")[1], "taken": t.timing_get("eig_warm_ok")[1],
+            "iterations": iters}
     t.timing(False)
     nt = (P + 63) // 64
     tiles = nt * (nt + 1) // 2
     g_ms = ops.get("gram", 0.0)
-    out = {"ms_per_iteration": ops, "P": P}
+    out = {"ms_per_iteration": ops, "P": P, "warm_eigen_step": warm}
     if g_ms > 0:
         executed = tiles * 64 * 64 * 2.0 * n   # MFMA flops issued (upper tiles, padded)
         algorithmic = 2.0 * n * P * P          # the full P x P product
@@ -869,8 +871,14 @@ def main():
         hooi_fixed = {"iters": r["iters"], "seconds": r["seconds"],
                       "s_per_iter": r["seconds"] / max(1, r["iters"]),
                       "final_error": float(r["errors"][-1]),
-                      "errors": [float(e) for e in r["errors"]]}
+                      "errors": [float(e) for e in r["errors"]],
+                      "iteration_times": [float(x) for x in np.diff(r["times"], prepend=0.0)]}
         out["%s_fixed_iterations" % aname] = hooi_fixed
+        if alg != 1:
+            # the same ops at the run's last factor, where the eigen-step's
+            # warm-started Rayleigh-Ritz form is taken when it converges
+            t.set_factor(r["U"])
+            out["dense_y_side_at_fixed_point"] = dense_y_side(t, N, n, R, w["delta"])
         out["%s_time_to_fit" % aname] = fit_study(t, alg, R, w["delta"], w["long_iters"])
 
     if not args.no_learned:

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace spamd {
 
@@ -317,6 +318,36 @@ __global__ void transpose_nr(const double *__restrict__ in, u64 rows, u64 cols,
   out[j * rows + i] = in[t];
 }
 
+// out[r][p] = C[r][p] * sqrt(W[p])  (C, out: R x P)
+__global__ void scale_cols_sqrt(const double *__restrict__ C, const double *__restrict__ W,
+                                u64 R, u64 P, double *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < R * P) out[t] = C[t] * sqrt(W[t % P]);
+}
+
+// res[r] = sum_p (GY[r][p] - theta[r] Y[r][p])^2: one 256-thread block per
+// row r, strided partial sums, then a fixed-order tree.
+__global__ void __launch_bounds__(256) resid_sq(const double *__restrict__ GY,
+                                                const double *__restrict__ Y,
+                                                const double *__restrict__ theta, u64 P,
+                                                double *__restrict__ res) {
+  __shared__ double s[256];
+  const u64 r = blockIdx.x;
+  const double th = theta[r];
+  double a = 0.0;
+  for (u64 p = threadIdx.x; p < P; p += 256) {
+    const double d = GY[r * P + p] - th * Y[r * P + p];
+    a += d * d;
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[r] = s[0];
+}
+
 // Columns from..R-1 of U (n x R) <- fixed pseudo-random values in [-1, 1)
 // (splitmix64 of the entry's position), the start of an orthonormal
 // completion when the span to orthonormalise has fewer than R dimensions.
@@ -487,6 +518,38 @@ u32 orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   return good;
 }
 
+// Host: L^{-1} (row-major, lower) of the Cholesky factor of the R x R Gram G
+// (row-major).  False when G is not safely positive definite (a pivot below
+// 1e-14 of the largest diagonal, or a non-finite diagonal).
+bool chol_inverse(const std::vector<double> &G, u32 R, std::vector<double> &Li) {
+  double dmax = 0.0;
+  for (u32 i = 0; i < R; ++i) dmax = std::max(dmax, G[(size_t)i * R + i]);
+  if (!(dmax > 0.0) || !std::isfinite(dmax)) return false;
+  // lower Cholesky factor L (row-major)
+  std::vector<double> L((size_t)R * R, 0.0);
+  for (u32 j = 0; j < R; ++j) {
+    double d = G[(size_t)j * R + j];
+    for (u32 k = 0; k < j; ++k) d -= L[(size_t)j * R + k] * L[(size_t)j * R + k];
+    if (!(d > dmax * 1e-14)) return false;
+    const double ljj = std::sqrt(d);
+    L[(size_t)j * R + j] = ljj;
+    for (u32 i = j + 1; i < R; ++i) {
+      double s = G[(size_t)i * R + j];
+      for (u32 k = 0; k < j; ++k) s -= L[(size_t)i * R + k] * L[(size_t)j * R + k];
+      L[(size_t)i * R + j] = s / ljj;
+    }
+  }
+  // L^{-1} by forward substitution on the identity
+  Li.assign((size_t)R * R, 0.0);
+  for (u32 col = 0; col < R; ++col)
+    for (u32 i = col; i < R; ++i) {
+      double s = (i == col) ? 1.0 : 0.0;
+      for (u32 k = col; k < i; ++k) s -= L[(size_t)i * R + k] * Li[(size_t)k * R + col];
+      Li[(size_t)i * R + col] = s / L[(size_t)i * R + i];
+    }
+  return true;
+}
+
 // One Cholesky-QR pass, U <- U L^{-T} with U^T U = L L^T (device Gram, host
 // Cholesky of the R x R Gram).  Two passes give Q of the thin QR with a
 // positive diagonal in R, orthonormal to working precision while cond(U) is
@@ -502,31 +565,8 @@ bool cholqr_pass(Ctx &c) {
   SP_HIP(hipMemcpyAsync(G.data(), g.p, G.size() * sizeof(double), hipMemcpyDeviceToHost,
                         c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
-  double dmax = 0.0;
-  for (u32 i = 0; i < R; ++i) dmax = std::max(dmax, G[(size_t)i * R + i]);
-  if (!(dmax > 0.0) || !std::isfinite(dmax)) return false;
-  // lower Cholesky factor L (row-major), in place
-  std::vector<double> L((size_t)R * R, 0.0);
-  for (u32 j = 0; j < R; ++j) {
-    double d = G[(size_t)j * R + j];
-    for (u32 k = 0; k < j; ++k) d -= L[(size_t)j * R + k] * L[(size_t)j * R + k];
-    if (!(d > dmax * 1e-14)) return false;
-    const double ljj = std::sqrt(d);
-    L[(size_t)j * R + j] = ljj;
-    for (u32 i = j + 1; i < R; ++i) {
-      double s = G[(size_t)i * R + j];
-      for (u32 k = 0; k < j; ++k) s -= L[(size_t)i * R + k] * L[(size_t)j * R + k];
-      L[(size_t)i * R + j] = s / ljj;
-    }
-  }
-  // Linv = L^{-1} (lower), by forward substitution on the identity
-  std::vector<double> Li((size_t)R * R, 0.0);
-  for (u32 col = 0; col < R; ++col)
-    for (u32 i = col; i < R; ++i) {
-      double s = (i == col) ? 1.0 : 0.0;
-      for (u32 k = col; k < i; ++k) s -= L[(size_t)i * R + k] * Li[(size_t)k * R + col];
-      Li[(size_t)i * R + col] = s / L[(size_t)i * R + i];
-    }
+  std::vector<double> Li;
+  if (!chol_inverse(G, R, Li)) return false;
   // Q = U L^{-T}: Q[i][r] = sum_s U[i][s] Linv[r][s], i.e. M = Linv for y_times_mt
   SP_HIP(hipMemcpyAsync(g.p, Li.data(), Li.size() * sizeof(double), hipMemcpyHostToDevice,
                         c.stream));
@@ -737,16 +777,137 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
   SP_CHECK_LAUNCH();
 }
 
+// Top-R eigenpairs of the Gram G (P x P, device, in range after
+// scale_to_unit) by a Rayleigh-Ritz step on span(X0), X0 = Z^T U =
+// diag(sqrt W) (U^T Y)^T with U the factor Y was computed from.  In the
+// Tucker loops U is the previous iteration's answer: X0's columns are G's
+// top-R eigenvectors times the singular values once the iteration has
+// reached its fixed point, and converge to them at the loop's own rate
+// before (one HOOI step is one subspace iteration from U).  The projected
+// problem H s = theta M s (H = X0^T G X0, M = X0^T X0, R x R) is solved on
+// the host (Cholesky of M, Jacobi).  Returns the largest Ritz residual
+// ||G y - theta y|| / theta_1 (infinity when X0 is rank-deficient or theta_R
+// is near the rank floor); V (P x R row-major) and theta (descending) hold
+// the Ritz pairs, which the caller keeps only when that residual is at the
+// accuracy of the direct solver.
+double eig_top_warm(Ctx &c, const double *G, u64 P, u32 R, double *V,
+                    std::vector<double> &theta) {
+  constexpr double kFail = std::numeric_limits<double>::infinity();
+  const u64 n = c.dim, RP = (u64)R * P, RR = (u64)R * R;
+  // Xt, GXt, Yt, GYt: [R][P]; small: H then M (R x R each), B (R x R);
+  // then theta and the residuals (R each)
+  c.ws_warm.reserve((4 * RP + 3 * RR + 2 * R) * sizeof(double));
+  double *Xt = c.ws_warm.as<double>();
+  double *GXt = Xt + RP;
+  double *Yt = GXt + RP;
+  double *GYt = Yt + RP;
+  double *small = GYt + RP;
+  double *Bd = small + 2 * RR;
+  double *th_d = Bd + RR;
+  double *res_d = th_d + R;
+  atb(c, c.d_U.as<double>(), R, c.d_Y.as<double>(), P, n, Yt, "warm_x0");  // U^T Y
+  scale_cols_sqrt<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, c.d_W.as<double>(), R, P, Xt);
+  SP_CHECK_LAUNCH();
+  y_times_mt<<<ymt_grid(R, P), 256, 0, c.stream>>>(Xt, R, P, G, P, GXt);  // rows of X0^T G
+  SP_CHECK_LAUNCH();
+  y_times_mt<<<ymt_grid(R, R), 256, 0, c.stream>>>(Xt, R, P, GXt, R, small);       // H
+  SP_CHECK_LAUNCH();
+  y_times_mt<<<ymt_grid(R, R), 256, 0, c.stream>>>(Xt, R, P, Xt, R, small + RR);   // M
+  SP_CHECK_LAUNCH();
+  std::vector<double> HM(2 * RR);
+  SP_HIP(hipMemcpyAsync(HM.data(), small, HM.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> M(HM.begin() + RR, HM.end()), Li;
+  for (u32 i = 0; i < R; ++i)
+    for (u32 j = 0; j < i; ++j)
+      M[(size_t)i * R + j] = M[(size_t)j * R + i] =
+          0.5 * (M[(size_t)i * R + j] + M[(size_t)j * R + i]);
+  if (!chol_inverse(M, R, Li)) return kFail;
+  // C = L^{-1} H L^{-T} (symmetrised), C = S diag(w) S^T
+  std::vector<double> T1(RR, 0.0), C(RR, 0.0);
+  for (u32 i = 0; i < R; ++i)
+    for (u32 j = 0; j < R; ++j) {
+      double v = 0.0;
+      for (u32 k = 0; k <= i; ++k) v += Li[(size_t)i * R + k] * HM[(size_t)k * R + j];
+      T1[(size_t)i * R + j] = v;
+    }
+  for (u32 i = 0; i < R; ++i)
+    for (u32 j = 0; j < R; ++j) {
+      double v = 0.0;
+      for (u32 k = 0; k <= j; ++k) v += T1[(size_t)i * R + k] * Li[(size_t)j * R + k];
+      C[(size_t)i * R + j] = v;
+    }
+  for (u32 i = 0; i < R; ++i)
+    for (u32 j = 0; j < i; ++j)
+      C[(size_t)i * R + j] = C[(size_t)j * R + i] =
+          0.5 * (C[(size_t)i * R + j] + C[(size_t)j * R + i]);
+  std::vector<double> w, S;
+  jacobi_eig(C, (int)R, w, S);
+  if (!(w[0] > 0.0) || !std::isfinite(w[0]) || !(w[R - 1] > w[0] * 1e-8)) return kFail;
+  // Ritz vectors y_r = X0 b_r with B = L^{-T} S: Yt = B^T Xt, G Yt = B^T GXt
+  std::vector<double> B(RR, 0.0);
+  for (u32 s = 0; s < R; ++s)
+    for (u32 r = 0; r < R; ++r) {
+      double v = 0.0;
+      for (u32 k = s; k < R; ++k) v += Li[(size_t)k * R + s] * S[(size_t)k * R + r];
+      B[(size_t)s * R + r] = v;
+    }
+  SP_HIP(hipMemcpyAsync(Bd, B.data(), RR * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  SP_HIP(hipMemcpyAsync(th_d, w.data(), R * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  atb(c, Bd, R, Xt, P, R, Yt, "warm_atb");
+  atb(c, Bd, R, GXt, P, R, GYt, "warm_atb");
+  resid_sq<<<R, 256, 0, c.stream>>>(GYt, Yt, th_d, P, res_d);
+  SP_CHECK_LAUNCH();
+  transpose_nr<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, R, P, V);
+  SP_CHECK_LAUNCH();
+  std::vector<double> res(R);
+  SP_HIP(hipMemcpyAsync(res.data(), res_d, R * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));  // also the point after which B and w may go
+  double worst = 0.0;
+  for (u32 r = 0; r < R; ++r) worst = std::max(worst, std::sqrt(res[r]));
+  theta.assign(w.begin(), w.end());
+  return worst / w[0];
+}
+
 void dense_leftsvd_y(Ctx &c, double *sigma) {
   dense_gram(c);
   const u64 P = c.P, n = c.dim;
   const u32 R = c.R;
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds packed width");
-  // top-R eigenvectors of G as V (P x R, row-major), eigenvalues descending
+  // top-R eigenvectors of G as V (P x R, row-major), eigenvalues descending:
+  // the warm-started Rayleigh-Ritz step when it converges, else the direct
+  // solver (both on G scaled into range; theta scaled back)
   DevBuf &V = c.ws_pr;
   V.reserve(P * R * sizeof(double));
   std::vector<double> theta;
-  eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+  const int ex = scale_to_unit(c, c.d_G.as<double>(), P * P);
+  bool warm = false;
+  if (c.warm_enabled && c.eig_method == 2) {
+    if (c.warm_skip > 0) {
+      --c.warm_skip;
+    } else {
+      double res;
+      {
+        ScopedTimer t(c, "eig_warm");
+        res = eig_top_warm(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+      }
+      // kept at the direct solver's accuracy; a near miss (the loop is
+      // converging onto its fixed point) is tried again at the next call,
+      // a clear miss backs off
+      warm = res <= 1e-13;
+      if (warm) {
+        ScopedTimer mark(c, "eig_warm_ok");  // counted success
+        c.warm_backoff = 0;
+      } else if (res > 1e-3) {
+        c.warm_backoff = std::min<u32>(4, std::max<u32>(1, 2 * c.warm_backoff));
+        c.warm_skip = c.warm_backoff;
+      }
+    }
+  }
+  if (!warm) eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+  for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(theta[r], -ex);
   c.ws_eig.reserve((2 * P + 2 + R) * sizeof(double) + 4 * sizeof(rocblas_int));
   double *theta_d = c.ws_eig.as<double>();
   if (sigma)

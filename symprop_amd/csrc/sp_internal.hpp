@@ -195,6 +195,13 @@ struct Ctx {
   // SP_TRD_SPIN at context creation sets it, to test the fallback)
   unsigned trd_spin_limit = 1u << 21;
   u64 trd_timeouts = 0;  // reductions that timed out and fell back
+  // leftsvd's warm-started Rayleigh-Ritz eigen-step (dense.hip): on by default
+  // (SP_EIG_WARM=0 at context creation turns it off); after an attempt that
+  // missed by far the next warm_skip calls go straight to the direct solver,
+  // with the skip doubling per such miss up to 4 and cleared by a success
+  bool warm_enabled = true;
+  u32 warm_skip = 0, warm_backoff = 0;
+  DevBuf ws_warm;  // its bases, products and small matrices
 
   // chunking of the work list for the current R: plan[0] with 64-owner
   // batches (generic, FP32 and BF16 kernels), plan[1] with the FP64 fast

@@ -368,12 +368,15 @@ SymTuckerDecomp SpSymtensor::TuckerHOOI(const Config &c) const {
   SymTuckerDecomp decomp(c, _dim, _order);
   auto &core = decomp._core;
   auto &factor = decomp._factor;
-  auto core_prev = core;
   init_factor(factor, c);
   const dim_t R = c.rank;
   const size_t P = symtens_size(R, _order - 1);
   const size_t full = (size_t)std::pow(R, _order - 1);
-  std::vector<real_t> core_p((size_t)R * P);
+  // the core stays packed inside the loop: the unpacked core's residual and
+  // norm are sums over the packed entries weighted by their multiplicities
+  // (R x P terms instead of R^N); it is unpacked once at the end
+  std::vector<real_t> core_p((size_t)R * P), core_prev((size_t)R * P, 0.0);
+  const std::vector<real_t> coeff = symtens_coeff(R, _order - 1);
   ck(sp_factor_upload(_ctx, factor.data(), R));
   IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
@@ -382,10 +385,8 @@ SymTuckerDecomp SpSymtensor::TuckerHOOI(const Config &c) const {
     timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
     timed("leftsvd", _ctx, [&] { ck(sp_leftsvd_y(_ctx, nullptr)); });
     timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
-    for (dim_t r = 0; r < R; ++r)
-      unpack_symtens(core_p.data() + (size_t)r * P, R, _order - 1, core.data() + r * full);
-    const real_t rel_res = rel_residual(core_prev, core);
-    const real_t core_norm = plain_norm(core);
+    const real_t rel_res = symtens_rel_res(core_prev, core_p, coeff, R);
+    const real_t core_norm = symtens_norm(core_p, coeff, R);
     const real_t error = fit_error(_norm, core_norm);  // reconstruct_error
     if (c.verbose)
       std::cout << std::scientific << std::setprecision(12) << "Iteration: " << iter
@@ -394,8 +395,10 @@ SymTuckerDecomp SpSymtensor::TuckerHOOI(const Config &c) const {
     const real_t error_diff = std::abs(prev_error - error);
     if (ctl.finish(error_diff)) break;
     prev_error = error;
-    core_prev = core;
+    core_prev = core_p;
   }
+  for (dim_t r = 0; r < R; ++r)
+    unpack_symtens(core_p.data() + (size_t)r * P, R, _order - 1, core.data() + r * full);
   ck(sp_factor_download(_ctx, factor.data()));
   decomp._fp64_sweeps = ctl.fp64_sweeps();
   decomp._seconds =

@@ -347,7 +347,9 @@ def spectrum_matrix(P, lam, seed):
 
 @pytest.mark.parametrize("eig", ["syevd", "syevj", "tridiag", "tridiag-rocsolver"])
 @pytest.mark.parametrize("P,R,kind", [(1, 1, "random"), (2, 1, "random"), (3, 2, "random"),
-                                      (50, 8, "random"), (220, 10, "gram"),
+                                      (32, 4, "random"), (33, 5, "gram"), (50, 8, "random"),
+                                      (97, 8, "random"), (130, 6, "clusters"),
+                                      (220, 10, "gram"), (240, 8, "rank3"), (256, 12, "gram"),
                                       (462, 6, "clusters"), (300, 8, "rank3"),
                                       (792, 8, "gram"), (1716, 8, "gram"),
                                       (2002, 8, "gram")])  # beyond the LDS-resident reduction
