@@ -344,7 +344,15 @@ s3ttmc_mfma(FastArgs a) {
   using acc_t = typename EL::acc_t;
   constexpr int T = M - H;
   constexpr int PH = psz(R, H), PT = psz(R, T), PM = psz(R, M);
-  constexpr int TH = (PH + 15) / 16, TT = (PT + 15) / 16;
+  // Tail strip (FP64): when the tail's last 16-wide tile would hold only 1-4
+  // real columns next to two full ones (P_T = 33..36: orders 5 and 6 at
+  // R = 8), the MFMAs cover the first 32 columns and those NSC columns are
+  // summed on the VALU from the same A fragments (the vector and matrix
+  // pipes run side by side): a third fewer MFMAs for 2 FMAs per fragment.
+  constexpr bool STRIP = sizeof(TE) == 8 && PT > 32 && PT <= 36;
+  constexpr int NSC = STRIP ? PT - 32 : 0;
+  constexpr int NSCW = (NSC + 1) / 2;  // strip columns of each wave of a pair
+  constexpr int TH = (PH + 15) / 16, TT = STRIP ? 2 : (PT + 15) / 16;
   constexpr int NTILE = TH * TT;
   constexpr int TPW = (NTILE + 3) / 4;
   constexpr int BO = kernel_bo<M, R, H, TE>();
@@ -426,6 +434,20 @@ s3ttmc_mfma(FastArgs a) {
   acc_t acc[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) acc[i] = acc_t{0, 0, 0, 0};
+  // Strip: with TT = 2 the waves of a pair (w, w^1) hold the A fragments of
+  // the same head tiles (tile column w % 2); wave w sums strip columns
+  // c0 .. c0 + NSCW - 1 for each of them.  Lane (kr, kc) accumulates head
+  // coefficient h = 16 th + kc over the owners 4 ks + kr; the four partial
+  // sums of an h are added across kr at the end of the row.
+  constexpr int NSL = BLOCKED ? BH : TPW;  // head tiles of a wave
+  const int c0 = (wave & 1) * NSCW;
+  TE strip[STRIP ? NSL : 1][STRIP ? NSCW : 1];
+  if constexpr (STRIP) {
+#pragma unroll
+    for (int u = 0; u < NSL; ++u)
+#pragma unroll
+      for (int j = 0; j < NSCW; ++j) strip[u][j] = TE(0);
+  }
 
   for (u64 b = ch.b_begin; b < ch.b_end; ++b) {
     const BatchV bt = read_batch(a.batches, b);
@@ -451,6 +473,13 @@ s3ttmc_mfma(FastArgs a) {
     for (int ks = 0; ks < ksteps; ++ks) {
       const TE *ar = sA + ks * 4 * LDA;
       const TE *br = sB + ks * 4 * LDB;
+      // this lane's owner's strip columns (zero past the strip)
+      TE bs[STRIP ? NSCW : 1];
+      if constexpr (STRIP) {
+#pragma unroll
+        for (int j = 0; j < NSCW; ++j)
+          bs[j] = c0 + j < NSC ? br[kr * LDB + 32 + c0 + j] : TE(0);
+      }
       if constexpr (BLOCKED) {
         // each fragment read once per k-step (accumulator i = u * BT + v)
         TE af[BH], bf[BT];
@@ -462,11 +491,23 @@ s3ttmc_mfma(FastArgs a) {
         for (int u = 0; u < BH; ++u)
 #pragma unroll
           for (int v = 0; v < BT; ++v) acc[u * BT + v] = EL::mma(af[u], bf[v], acc[u * BT + v]);
+        if constexpr (STRIP) {
+#pragma unroll
+          for (int u = 0; u < BH; ++u)
+#pragma unroll
+            for (int j = 0; j < NSCW; ++j) strip[u][j] = fma(af[u], bs[j], strip[u][j]);
+        }
       } else {
 #pragma unroll
         for (int i = 0; i < TPW; ++i) {
-          if (NTILE % 4 == 0 || wave + 4 * i < NTILE)
-            acc[i] = EL::mma(ar[aoff[i]], br[boff[i]], acc[i]);
+          if (NTILE % 4 == 0 || wave + 4 * i < NTILE) {
+            const TE av = ar[aoff[i]];
+            acc[i] = EL::mma(av, br[boff[i]], acc[i]);
+            if constexpr (STRIP) {
+#pragma unroll
+              for (int j = 0; j < NSCW; ++j) strip[i][j] = fma(av, bs[j], strip[i][j]);
+            }
+          }
         }
       }
     }
@@ -486,6 +527,25 @@ s3ttmc_mfma(FastArgs a) {
         }
       }
       acc[i] = acc_t{0, 0, 0, 0};
+    }
+    if constexpr (STRIP) {
+      // strip column c0 + j of head h = 16 th + kc: the four owner classes
+      // kr added in a fixed order ((0 + 1) + (2 + 3) in every lane)
+#pragma unroll
+      for (int u = 0; u < NSL; ++u) {
+        int th, tt;
+        tile_of(BLOCKED ? u * BT : u, th, tt);
+        const bool valid = BLOCKED || wave + 4 * u < NTILE;
+        const int h = th * 16 + kc;
+#pragma unroll
+        for (int j = 0; j < NSCW; ++j) {
+          double v = strip[u][j];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          if (valid && kr == 0 && h < PH && c0 + j < NSC) sF[h * PT + 32 + c0 + j] = v;
+          strip[u][j] = TE(0);
+        }
+      }
     }
     __syncthreads();
     double *out = (ch.partial >= 0) ? a.partial + (u64)ch.partial * PM : a.Y + (u64)bt.row * PM;
