@@ -344,14 +344,13 @@ s3ttmc_mfma(FastArgs a) {
   using acc_t = typename EL::acc_t;
   constexpr int T = M - H;
   constexpr int PH = psz(R, H), PT = psz(R, T), PM = psz(R, M);
-  // Tail strip (FP64): when the tail's last 16-wide tile would hold only 1-4
-  // real columns next to two full ones (P_T = 33..36: orders 5 and 6 at
-  // R = 8), the MFMAs cover the first 32 columns and those NSC columns are
-  // summed on the VALU from the same A fragments (the vector and matrix
-  // pipes run side by side): a third fewer MFMAs for 2 FMAs per fragment.
-  constexpr bool STRIP = sizeof(TE) == 8 && PT > 32 && PT <= 36;
+  // Tail strip: when the tail's last 16-wide tile would hold only 1-4 real
+  // columns next to two full ones (P_T = 33..36: orders 5 and 6 at R = 8),
+  // the MFMAs cover the first 32 columns and those NSC columns are summed on
+  // the VALU from the same A fragments (the vector and matrix pipes run side
+  // by side): a third fewer MFMAs for a few FMAs per fragment.
+  constexpr bool STRIP = PT > 32 && PT <= 36;
   constexpr int NSC = STRIP ? PT - 32 : 0;
-  constexpr int NSCW = (NSC + 1) / 2;  // strip columns of each wave of a pair
   constexpr int TH = (PH + 15) / 16, TT = STRIP ? 2 : (PT + 15) / 16;
   constexpr int NTILE = TH * TT;
   constexpr int TPW = (NTILE + 3) / 4;
@@ -434,13 +433,17 @@ s3ttmc_mfma(FastArgs a) {
   acc_t acc[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) acc[i] = acc_t{0, 0, 0, 0};
-  // Strip: with TT = 2 the waves of a pair (w, w^1) hold the A fragments of
-  // the same head tiles (tile column w % 2); wave w sums strip columns
-  // c0 .. c0 + NSCW - 1 for each of them.  Lane (kr, kc) accumulates head
-  // coefficient h = 16 th + kc over the owners 4 ks + kr; the four partial
-  // sums of an h are added across kr at the end of the row.
+  // Strip: with TT = 2, round-robin tiles or 2 x 2 rectangles give the waves
+  // of a pair (w, w^1) the A fragments of the same head tiles (tile column
+  // w % 2), and wave w sums half the strip columns, c0 .. c0 + NSCW - 1, for
+  // each of them; with 4 x 1 rectangles every wave holds its own head tiles
+  // and sums all strip columns.  Lane (kr, kc) accumulates head coefficient
+  // h = 16 th + kc over the owners 4 ks + kr; the four partial sums of an h
+  // are added across kr at the end of the row.
+  constexpr bool PAIR = !BLOCKED || SPL.st == 2;
+  constexpr int NSCW = PAIR ? (NSC + 1) / 2 : NSC;  // strip columns of a wave
   constexpr int NSL = BLOCKED ? BH : TPW;  // head tiles of a wave
-  const int c0 = (wave & 1) * NSCW;
+  const int c0 = PAIR ? (wave & 1) * NSCW : 0;
   TE strip[STRIP ? NSL : 1][STRIP ? NSCW : 1];
   if constexpr (STRIP) {
 #pragma unroll
