@@ -251,13 +251,14 @@ __device__ __forceinline__ void load_rows(const T *U, const u32 (&tp)[M], T (&ro
 //    J / 256 in bits 28..31, the Ypre offset h * P_T + t below, slot 15
 //    pads), coalesced loads that do not depend on one another.
 // Measured on MI355X: the thread-major form makes the BF16 kernel 4-11%
-// faster on orders 4-7 and the order-7 FP64 kernel (C(6, 3) = 20 terms per
-// output) 4% faster; with short lists it costs the FP64 kernels registers
+// faster on orders 4-7, the order-7 FP64 kernel (C(6, 3) = 20 terms per
+// output) 4% and the order-6 FP64 kernel (C(5, 3) = 10 terms, with the tail
+// strip) 2% faster; with shorter lists it costs the FP64 kernels registers
 // (the order-5 FP64 kernel went from 128 to 152 VGPRs and 9% slower) and the
 // FP32 kernels gain nothing, so those keep the CSR form.
 template <int M, int H, class TE>
 constexpr bool fold_thread_major() {
-  return sizeof(TE) == 2 || (sizeof(TE) == 8 && cbinom(M, H) >= 20);
+  return sizeof(TE) == 2 || (sizeof(TE) == 8 && cbinom(M, H) >= 10);
 }
 
 template <int M, int H, int PM, class TE, class TF>
