@@ -21,6 +21,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <limits>
 
@@ -346,6 +347,17 @@ __global__ void __launch_bounds__(256) resid_sq(const double *__restrict__ GY,
     __syncthreads();
   }
   if (threadIdx.x == 0) res[r] = s[0];
+}
+
+// out[r][p] = (GY[r][p] - theta[r] Y[r][p]) * inv[r]: Ritz residual
+// directions, each scaled by the inverse of its norm
+__global__ void ritz_dir(const double *__restrict__ GY, const double *__restrict__ Y,
+                         const double *__restrict__ theta, const double *__restrict__ inv,
+                         u64 R, u64 P, double *__restrict__ out) {
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R * P) return;
+  const u64 r = t / P;
+  out[t] = (GY[t] - theta[r] * Y[t]) * inv[r];
 }
 
 // Columns from..R-1 of U (n x R) <- fixed pseudo-random values in [-1, 1)
@@ -777,98 +789,138 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
   SP_CHECK_LAUNCH();
 }
 
-// Top-R eigenpairs of the Gram G (P x P, device, in range after
-// scale_to_unit) by a Rayleigh-Ritz step on span(X0), X0 = Z^T U =
-// diag(sqrt W) (U^T Y)^T with U the factor Y was computed from.  In the
-// Tucker loops U is the previous iteration's answer: X0's columns are G's
-// top-R eigenvectors times the singular values once the iteration has
-// reached its fixed point, and converge to them at the loop's own rate
-// before (one HOOI step is one subspace iteration from U).  The projected
-// problem H s = theta M s (H = X0^T G X0, M = X0^T X0, R x R) is solved on
-// the host (Cholesky of M, Jacobi).  Returns the largest Ritz residual
-// ||G y - theta y|| / theta_1 (infinity when X0 is rank-deficient or theta_R
-// is near the rank floor); V (P x R row-major) and theta (descending) hold
-// the Ritz pairs, which the caller keeps only when that residual is at the
-// accuracy of the direct solver.
-double eig_top_warm(Ctx &c, const double *G, u64 P, u32 R, double *V,
-                    std::vector<double> &theta) {
+// Generalised Rayleigh-Ritz on the span of the k rows of Kt (k x P, device),
+// GKt = Kt G: H = Kt G Kt^T and M = Kt Kt^T on the device, H s = theta M s
+// on the host (Cholesky of M, Jacobi), then the top R Ritz vectors as the
+// rows of Yt, G Yt in GYt and their squared residuals ||G y - theta y||^2 in
+// res (host).  small: 3 k^2 doubles, th_d: R, res_d: R (device).  Returns
+// the largest residual / theta_1, or infinity when M is not safely positive
+// definite or theta_R is not above 1e-8 theta_1.
+double rayleigh_ritz(Ctx &c, const double *Kt, const double *GKt, u32 k, u64 P, u32 R,
+                     double *small, double *th_d, double *res_d, double *Yt, double *GYt,
+                     std::vector<double> &theta, std::vector<double> &res) {
   constexpr double kFail = std::numeric_limits<double>::infinity();
-  const u64 n = c.dim, RP = (u64)R * P, RR = (u64)R * R;
-  // Xt, GXt, Yt, GYt: [R][P]; small: H then M (R x R each), B (R x R);
-  // then theta and the residuals (R each)
-  c.ws_warm.reserve((4 * RP + 3 * RR + 2 * R) * sizeof(double));
-  double *Xt = c.ws_warm.as<double>();
-  double *GXt = Xt + RP;
-  double *Yt = GXt + RP;
-  double *GYt = Yt + RP;
-  double *small = GYt + RP;
-  double *Bd = small + 2 * RR;
-  double *th_d = Bd + RR;
-  double *res_d = th_d + R;
-  atb(c, c.d_U.as<double>(), R, c.d_Y.as<double>(), P, n, Yt, "warm_x0");  // U^T Y
-  scale_cols_sqrt<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, c.d_W.as<double>(), R, P, Xt);
+  const u64 kk = (u64)k * k;
+  y_times_mt<<<ymt_grid(k, k), 256, 0, c.stream>>>(Kt, k, P, GKt, k, small);     // H
   SP_CHECK_LAUNCH();
-  y_times_mt<<<ymt_grid(R, P), 256, 0, c.stream>>>(Xt, R, P, G, P, GXt);  // rows of X0^T G
+  y_times_mt<<<ymt_grid(k, k), 256, 0, c.stream>>>(Kt, k, P, Kt, k, small + kk); // M
   SP_CHECK_LAUNCH();
-  y_times_mt<<<ymt_grid(R, R), 256, 0, c.stream>>>(Xt, R, P, GXt, R, small);       // H
-  SP_CHECK_LAUNCH();
-  y_times_mt<<<ymt_grid(R, R), 256, 0, c.stream>>>(Xt, R, P, Xt, R, small + RR);   // M
-  SP_CHECK_LAUNCH();
-  std::vector<double> HM(2 * RR);
+  std::vector<double> HM(2 * kk);
   SP_HIP(hipMemcpyAsync(HM.data(), small, HM.size() * sizeof(double), hipMemcpyDeviceToHost,
                         c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));
-  std::vector<double> M(HM.begin() + RR, HM.end()), Li;
-  for (u32 i = 0; i < R; ++i)
-    for (u32 j = 0; j < i; ++j)
-      M[(size_t)i * R + j] = M[(size_t)j * R + i] =
-          0.5 * (M[(size_t)i * R + j] + M[(size_t)j * R + i]);
-  if (!chol_inverse(M, R, Li)) return kFail;
+  auto sym = [k](double *A) {
+    for (u32 i = 0; i < k; ++i)
+      for (u32 j = 0; j < i; ++j)
+        A[(size_t)i * k + j] = A[(size_t)j * k + i] =
+            0.5 * (A[(size_t)i * k + j] + A[(size_t)j * k + i]);
+  };
+  sym(HM.data());
+  sym(HM.data() + kk);
+  std::vector<double> M(HM.begin() + kk, HM.end()), Li;
+  if (!chol_inverse(M, k, Li)) return kFail;
   // C = L^{-1} H L^{-T} (symmetrised), C = S diag(w) S^T
-  std::vector<double> T1(RR, 0.0), C(RR, 0.0);
-  for (u32 i = 0; i < R; ++i)
-    for (u32 j = 0; j < R; ++j) {
+  std::vector<double> T1(kk, 0.0), C(kk, 0.0);
+  for (u32 i = 0; i < k; ++i)
+    for (u32 j = 0; j < k; ++j) {
       double v = 0.0;
-      for (u32 k = 0; k <= i; ++k) v += Li[(size_t)i * R + k] * HM[(size_t)k * R + j];
-      T1[(size_t)i * R + j] = v;
+      for (u32 q = 0; q <= i; ++q) v += Li[(size_t)i * k + q] * HM[(size_t)q * k + j];
+      T1[(size_t)i * k + j] = v;
     }
-  for (u32 i = 0; i < R; ++i)
-    for (u32 j = 0; j < R; ++j) {
+  for (u32 i = 0; i < k; ++i)
+    for (u32 j = 0; j < k; ++j) {
       double v = 0.0;
-      for (u32 k = 0; k <= j; ++k) v += T1[(size_t)i * R + k] * Li[(size_t)j * R + k];
-      C[(size_t)i * R + j] = v;
+      for (u32 q = 0; q <= j; ++q) v += T1[(size_t)i * k + q] * Li[(size_t)j * k + q];
+      C[(size_t)i * k + j] = v;
     }
-  for (u32 i = 0; i < R; ++i)
-    for (u32 j = 0; j < i; ++j)
-      C[(size_t)i * R + j] = C[(size_t)j * R + i] =
-          0.5 * (C[(size_t)i * R + j] + C[(size_t)j * R + i]);
+  sym(C.data());
   std::vector<double> w, S;
-  jacobi_eig(C, (int)R, w, S);
+  jacobi_eig(C, (int)k, w, S);
   if (!(w[0] > 0.0) || !std::isfinite(w[0]) || !(w[R - 1] > w[0] * 1e-8)) return kFail;
-  // Ritz vectors y_r = X0 b_r with B = L^{-T} S: Yt = B^T Xt, G Yt = B^T GXt
-  std::vector<double> B(RR, 0.0);
-  for (u32 s = 0; s < R; ++s)
+  // Ritz vectors y_r = Kt^T b_r with B = L^{-T} S (first R columns): Yt = B^T Kt
+  std::vector<double> B((size_t)k * R, 0.0);
+  for (u32 s = 0; s < k; ++s)
     for (u32 r = 0; r < R; ++r) {
       double v = 0.0;
-      for (u32 k = s; k < R; ++k) v += Li[(size_t)k * R + s] * S[(size_t)k * R + r];
+      for (u32 q = s; q < k; ++q) v += Li[(size_t)q * k + s] * S[(size_t)q * k + r];
       B[(size_t)s * R + r] = v;
     }
-  SP_HIP(hipMemcpyAsync(Bd, B.data(), RR * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  double *Bd = small + 2 * kk;
+  SP_HIP(hipMemcpyAsync(Bd, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice,
+                        c.stream));
   SP_HIP(hipMemcpyAsync(th_d, w.data(), R * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  atb(c, Bd, R, Xt, P, R, Yt, "warm_atb");
-  atb(c, Bd, R, GXt, P, R, GYt, "warm_atb");
+  atb(c, Bd, R, Kt, P, k, Yt, "warm_atb");
+  atb(c, Bd, R, GKt, P, k, GYt, "warm_atb");
   resid_sq<<<R, 256, 0, c.stream>>>(GYt, Yt, th_d, P, res_d);
   SP_CHECK_LAUNCH();
-  transpose_nr<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, R, P, V);
-  SP_CHECK_LAUNCH();
-  std::vector<double> res(R);
+  res.resize(R);
   SP_HIP(hipMemcpyAsync(res.data(), res_d, R * sizeof(double), hipMemcpyDeviceToHost,
                         c.stream));
   SP_HIP(hipStreamSynchronize(c.stream));  // also the point after which B and w may go
   double worst = 0.0;
   for (u32 r = 0; r < R; ++r) worst = std::max(worst, std::sqrt(res[r]));
-  theta.assign(w.begin(), w.end());
+  theta.assign(w.begin(), w.begin() + R);
   return worst / w[0];
+}
+
+// Top-R eigenpairs of the Gram G (P x P, device, in range after
+// scale_to_unit) by Rayleigh-Ritz on span(X0), X0 = Z^T U =
+// diag(sqrt W) (U^T Y)^T with U the factor Y was computed from, and, when
+// that misses by less than 1e-3, on span(X0, R0) with R0 the first step's
+// Ritz residuals G y - theta y (orthogonal to span(X0); one block Krylov
+// step).  In the Tucker loops U is the previous iteration's answer: X0's
+// columns are G's top-R eigenvectors times the singular values once the
+// iteration has reached its fixed point, and converge to them at the loop's
+// own rate before (one HOOI step is one subspace iteration from U).  Returns
+// the largest Ritz residual ||G y - theta y|| / theta_1 of the last step
+// (infinity when a basis is rank-deficient or theta_R is near the rank
+// floor); V (P x R row-major) and theta (descending) hold its Ritz pairs,
+// which the caller keeps only when that residual is at the accuracy of the
+// direct solver.
+double eig_top_warm(Ctx &c, const double *G, u64 P, u32 R, double *V,
+                    std::vector<double> &theta) {
+  const u64 n = c.dim, RP = (u64)R * P, RR = (u64)R * R;
+  // Xt, GXt, Yt, GYt: [R][P]; Kt, GKt: [2R][P]; Y2, GY2: [R][P]; small:
+  // 3 (2R)^2; theta, residuals, inverse residual norms: R each
+  c.ws_warm.reserve((10 * RP + 12 * RR + 3 * R) * sizeof(double));
+  double *Xt = c.ws_warm.as<double>();
+  double *GXt = Xt + RP;
+  double *Yt = GXt + RP;
+  double *GYt = Yt + RP;
+  double *Kt = GYt + RP;
+  double *GKt = Kt + 2 * RP;
+  double *Y2 = GKt + 2 * RP;
+  double *GY2 = Y2 + RP;
+  double *small = GY2 + RP;
+  double *th_d = small + 12 * RR;
+  double *res_d = th_d + R;
+  double *inv_d = res_d + R;
+  atb(c, c.d_U.as<double>(), R, c.d_Y.as<double>(), P, n, Yt, "warm_x0");  // U^T Y
+  scale_cols_sqrt<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, c.d_W.as<double>(), R, P, Xt);
+  SP_CHECK_LAUNCH();
+  y_times_mt<<<ymt_grid(R, P), 256, 0, c.stream>>>(Xt, R, P, G, P, GXt);  // rows of X0^T G
+  SP_CHECK_LAUNCH();
+  std::vector<double> res;
+  double worst = rayleigh_ritz(c, Xt, GXt, R, P, R, small, th_d, res_d, Yt, GYt, theta, res);
+  const double *Yfin = Yt;
+  if (worst > 1e-13 && worst < 1e-3 && 2 * (u64)R <= P) {
+    // second block: the Ritz residual directions, each scaled to unit length
+    std::vector<double> inv(R);
+    for (u32 r = 0; r < R; ++r) inv[r] = 1.0 / std::max(std::sqrt(res[r]), 1e-300);
+    SP_HIP(hipMemcpyAsync(inv_d, inv.data(), R * sizeof(double), hipMemcpyHostToDevice,
+                          c.stream));
+    SP_HIP(hipMemcpyAsync(Kt, Yt, RP * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    SP_HIP(hipMemcpyAsync(GKt, GYt, RP * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    ritz_dir<<<grid_for(RP, 256), 256, 0, c.stream>>>(GYt, Yt, th_d, inv_d, R, P, Kt + RP);
+    SP_CHECK_LAUNCH();
+    y_times_mt<<<ymt_grid(R, P), 256, 0, c.stream>>>(Kt + RP, R, P, G, P, GKt + RP);
+    SP_CHECK_LAUNCH();
+    worst = rayleigh_ritz(c, Kt, GKt, 2 * R, P, R, small, th_d, res_d, Y2, GY2, theta, res);
+    Yfin = Y2;  // inv is read by the upload above; rayleigh_ritz synchronised
+  }
+  transpose_nr<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yfin, R, P, V);
+  SP_CHECK_LAUNCH();
+  return worst;
 }
 
 void dense_leftsvd_y(Ctx &c, double *sigma) {
@@ -884,29 +936,44 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   std::vector<double> theta;
   const int ex = scale_to_unit(c, c.d_G.as<double>(), P * P);
   bool warm = false;
+  auto ms_since = [](std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+        .count();
+  };
+  auto mean = [](double m, double x) { return m > 0.0 ? 0.75 * m + 0.25 * x : x; };
   if (c.warm_enabled && c.eig_method == 2) {
     if (c.warm_skip > 0) {
       --c.warm_skip;
     } else {
       double res;
+      const auto t0 = std::chrono::steady_clock::now();
       {
         ScopedTimer t(c, "eig_warm");
         res = eig_top_warm(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
       }
-      // kept at the direct solver's accuracy; a near miss (the loop is
-      // converging onto its fixed point) is tried again at the next call,
-      // a clear miss backs off
+      // kept at the direct solver's accuracy.  A near miss (the loop is
+      // closing in on its fixed point) is tried again at the next call;
+      // after a clear miss the next calls skip the attempt, as many as keep
+      // the misses within a twentieth of the direct path's time
       warm = res <= 1e-13;
       if (warm) {
         ScopedTimer mark(c, "eig_warm_ok");  // counted success
-        c.warm_backoff = 0;
-      } else if (res > 1e-3) {
-        c.warm_backoff = std::min<u32>(4, std::max<u32>(1, 2 * c.warm_backoff));
-        c.warm_skip = c.warm_backoff;
+      } else {
+        c.warm_miss_ms = mean(c.warm_miss_ms, ms_since(t0));
+        if (res > 1e-6)
+          c.warm_skip = c.direct_eig_ms > 0.0
+                            ? (u32)std::min(64.0, std::ceil(c.warm_miss_ms /
+                                                            (0.05 * c.direct_eig_ms)) - 1.0)
+                            : 1;
       }
     }
   }
-  if (!warm) eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+  if (!warm) {
+    const auto t0 = std::chrono::steady_clock::now();
+    eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+    SP_HIP(hipStreamSynchronize(c.stream));
+    c.direct_eig_ms = mean(c.direct_eig_ms, ms_since(t0));
+  }
   for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(theta[r], -ex);
   c.ws_eig.reserve((2 * P + 2 + R) * sizeof(double) + 4 * sizeof(rocblas_int));
   double *theta_d = c.ws_eig.as<double>();

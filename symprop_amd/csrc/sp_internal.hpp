@@ -196,11 +196,13 @@ struct Ctx {
   unsigned trd_spin_limit = 1u << 21;
   u64 trd_timeouts = 0;  // reductions that timed out and fell back
   // leftsvd's warm-started Rayleigh-Ritz eigen-step (dense.hip): on by default
-  // (SP_EIG_WARM=0 at context creation turns it off); after an attempt that
-  // missed by far the next warm_skip calls go straight to the direct solver,
-  // with the skip doubling per such miss up to 4 and cleared by a success
+  // (SP_EIG_WARM=0 at context creation turns it off).  After a miss the next
+  // warm_skip calls go straight to the direct solver, sized from the measured
+  // times (running means, ms) so that misses cost at most a twentieth of the
+  // direct path's time; a near miss or a success clears it
   bool warm_enabled = true;
-  u32 warm_skip = 0, warm_backoff = 0;
+  u32 warm_skip = 0;
+  double warm_miss_ms = 0.0, direct_eig_ms = 0.0;
   DevBuf ws_warm;  // its bases, products and small matrices
 
   // chunking of the work list for the current R: plan[0] with 64-owner
