@@ -380,6 +380,79 @@ def cpu_hooi_baseline(inds, vals, order, dim, R, max_iter, delta, budget_s=40.0,
                       "iterations from the reference's random start" % (threads, len(errs))}
 
 
+def cpu_hooi_chunked(inds, vals, order, dim, R, iters, delta, chunk_nnz):
+    """HOOI on the host cores, measured end to end, for a tensor whose opt5
+    level arrays do not fit host memory in one piece (the order-6 planted
+    tensor's need about 300 GB).  The nonzeros (sorted) are cut into
+    contiguous chunks of chunk_nnz; each chunk's tree is built once by the
+    reference's own DTree + CompressedDTree (oracle/_ref), and each iteration
+    runs the reference's HOOI loop (spsymtensor.cpp:201-252): regularize,
+    Y = sum over the chunks of the reference's kronecker_product_opt5 (S3TTMc
+    is linear in the tensor) on all host cores, then the leading left
+    singular vectors of the unpacked Y through the packed Gram (numpy eigh of
+    G = Z^T Z, Z = Y diag(sqrt W): the n x 32,768 unpacked Y of the literal
+    dgesvd would need 13 GB per iteration), the core and the error.  Same
+    random start as the GPU run (mt19937 + normal, seed 0)."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    from oracle import ref
+    from oracle import tucker
+    if not ref.available():
+        return {"error": "oracle/_ref not built"}
+    threads = host_threads()
+    P = binom(R + order - 2, order - 1)
+    W = np.array(oracle_coeff(R, order - 1))
+    sw = np.sqrt(W)
+    norm = oracle.norm(order, inds, vals)
+    t0 = time.perf_counter()
+    trees = []
+    for c0 in range(0, len(vals), chunk_nnz):
+        a = np.ascontiguousarray(inds[c0:c0 + chunk_nnz], dtype=np.uint32)
+        v = np.ascontiguousarray(vals[c0:c0 + chunk_nnz])
+        tb = time.perf_counter()
+        trees.append(ref.lib().ref_build_quiet(order, dim, len(v), a.ctypes.data_as(ref.u32p),
+                                               v.ctypes.data_as(ref.f64p)))
+        log("cpu chunked HOOI: chunk %d (%d nonzeros) built in %.1f s"
+            % (len(trees), len(v), time.perf_counter() - tb))
+    build_s = time.perf_counter() - t0
+    U = oracle.random_factor(0, dim, R)
+    errs, times, s3_times = [], [], []
+    try:
+        for it in range(iters):
+            ti = time.perf_counter()
+            U = tucker.regularize(U, delta)
+            Uc = np.ascontiguousarray(U)
+            Y = np.zeros((dim, P))
+            Yc = np.zeros((dim, P))
+            ts = time.perf_counter()
+            for h in trees:
+                ref.lib().ref_opt5(h, Uc.ctypes.data_as(ref.f64p), R,
+                                   Yc.ctypes.data_as(ref.f64p), P, threads)
+                Y += Yc
+            s3_times.append(time.perf_counter() - ts)
+            Z = Y * sw
+            lam, V = np.linalg.eigh(Z.T @ Z)
+            V, lam = V[:, ::-1][:, :R], lam[::-1][:R]
+            U = Z @ V / np.sqrt(lam)
+            core = U.T @ Y
+            err = tucker.fit_error(norm, np.sqrt((core ** 2 * W).sum()))
+            errs.append(float(err))
+            times.append(time.perf_counter() - ti)
+            log("cpu chunked HOOI: iteration %d %.1f s (S3TTMc %.1f s), error %.12f"
+                % (it + 1, times[-1], s3_times[-1], err))
+    finally:
+        for h in trees:
+            ref.lib().ref_free(h)
+    return {"kind": "reference", "cores": threads, "chunks": len(trees),
+            "chunk_nnz": chunk_nnz, "construction_s": build_s, "iters": len(errs),
+            "s_per_iter": float(np.mean(times)), "iteration_s": times,
+            "s3ttmc_s": s3_times, "errors": errs,
+            "sample": "whole tensor in %d contiguous chunks of %d sorted nonzeros: the "
+                      "reference's tree per chunk (built once) and kronecker_product_opt5 "
+                      "summed over the chunks with %d OpenMP threads per iteration; numpy "
+                      "packed-Gram eigen-step" % (len(trees), chunk_nnz, threads)}
+
+
 def dense_y_side(t, N, n, R, delta, iters=3):
     """One HOOI iteration's device work, op by op (HIP events on the library
     stream), and the MFMA utilisation of the Y-side contractions: the Gram
@@ -725,6 +798,11 @@ def main():
     ap.add_argument("--learned-hoqri-nnz", type=int, default=10_000_000)
     ap.add_argument("--learned-hoqri-train", type=int, default=16)
     ap.add_argument("--learned-hoqri-test", type=int, default=4)
+    ap.add_argument("--cpu-hooi-chunked", type=int, default=0, metavar="ITERS",
+                    help="only measure ITERS iterations of the host-core HOOI with the "
+                         "reference's opt5 over contiguous chunks of the workload, next to "
+                         "the same iterations on the GPU, and exit")
+    ap.add_argument("--chunk-nnz", type=int, default=600_000)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(reference for HBM counter scaling) and exit")
@@ -760,6 +838,21 @@ def main():
     inds, vals = sp.generate(w["kind"], N, n, w["nnz"], w["param"], w["communities"], 0)
     nnz = len(vals)
     log("nnz", nnz)
+    if args.cpu_hooi_chunked:
+        tg = sp.SpSymtensor(n, N, inds, vals)
+        r = tg.tucker(alg=0, rank=R, max_iter=args.cpu_hooi_chunked, rel_tol=0.0,
+                      delta=w["delta"], seed=0)
+        del tg
+        cb = cpu_hooi_chunked(inds, vals, N, n, R, args.cpu_hooi_chunked, w["delta"],
+                              args.chunk_nnz)
+        if "errors" in cb:
+            cb["gpu_fp64_s_per_iter"] = r["seconds"] / max(1, r["iters"])
+            cb["gpu_errors"] = [float(e) for e in r["errors"]]
+            cb["error_max_diff_vs_gpu"] = float(np.max(np.abs(
+                np.array(cb["errors"]) - r["errors"][:len(cb["errors"])])))
+            cb["gpu_speedup_per_iteration"] = cb["s_per_iter"] / cb["gpu_fp64_s_per_iter"]
+        print(json.dumps({"workload": args.workload, "cpu_hooi_chunked": cb}), flush=True)
+        return
 
     def build_parts():
         """(upload + device checks + norm, S3TTMc-ready build, interior levels)."""
