@@ -518,15 +518,15 @@ u32 orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
         M[(size_t)r * R + s] = v;
       }
     }
-  SP_HIP(hipMemcpyAsync(g.p, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice,
-                        c.stream));
+  c.h_small.swap(M);
+  SP_HIP(hipMemcpyAsync(g.p, c.h_small.data(), c.h_small.size() * sizeof(double),
+                        hipMemcpyHostToDevice, c.stream));
   tmp.reserve(n * R * sizeof(double));
   y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
                                                    g.as<double>(), R, tmp.as<double>());
   SP_CHECK_LAUNCH();
   SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
                         c.stream));
-  SP_HIP(hipStreamSynchronize(c.stream));
   return good;
 }
 
@@ -663,9 +663,13 @@ void dense_gram(Ctx &c) {
   const u64 rows_per = ((n + nk - 1) / nk + GK - 1) / GK * GK;
   nk = (n + rows_per - 1) / rows_per;
   DevBuf &dt = c.ws_tiles;
-  dt.reserve(ntiles * sizeof(int2));
-  SP_HIP(hipMemcpyAsync(dt.p, tiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice,
-                        c.stream));
+  if (c.gram_tiles_P != P || dt.bytes < ntiles * sizeof(int2)) {
+    dt.reserve(ntiles * sizeof(int2));
+    SP_HIP(hipMemcpyAsync(dt.p, tiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice,
+                          c.stream));
+    SP_HIP(hipStreamSynchronize(c.stream));  // tiles is local
+    c.gram_tiles_P = P;
+  }
   c.d_scratch.reserve(nk * ntiles * GT * GT * sizeof(double));
   c.d_G.reserve(P * P * sizeof(double));
   {
@@ -678,7 +682,6 @@ void dense_gram(Ctx &c) {
         c.d_G.as<double>());
     SP_CHECK_LAUNCH();
   }
-  SP_HIP(hipStreamSynchronize(c.stream));
 }
 
 // All eigenpairs of the P x P symmetric G by rocSOLVER (syevd, or syevj when
@@ -708,19 +711,37 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
     Vr = c.ws_trd_v.as<double>();
   }
   bool lower;
+  std::vector<double> &hd = c.h_eig_de, &Z = c.h_eig_z;
+  hd.resize(2 * P);
+  Z.resize((u64)R * P);
   {
     ScopedTimer t(c, "sytrd");
-    const u64 timeouts = c.trd_timeouts;
-    lower = Vr && sytrd_lower(c, G, P, D, E, tau, Vr);
-    if (c.trd_timeouts != timeouts) ScopedTimer mark(c, "sytrd_timeout");  // counted fallback
-    if (!lower && rocsolver_dsytrd(h, rocblas_fill_upper, (rocblas_int)P, G, (rocblas_int)P,
-                                   D, E, tau) != rocblas_status_success)
-      throw Error(SP_ERR_DEVICE, "rocsolver_dsytrd failed");
+    const int *abort_flag = nullptr;
+    lower = Vr && sytrd_lower(c, G, P, D, E, tau, Vr, &abort_flag);
+    if (lower) {
+      // the reduction's output and its time-out flag in one wait
+      SP_HIP(hipMemcpyAsync(hd.data(), D, 2 * P * sizeof(double), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipMemcpyAsync(&c.h_trd_abort, abort_flag, sizeof(int), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipStreamSynchronize(c.stream));
+      if (c.h_trd_abort) {
+        // some workgroup was not resident in time (the device is shared with
+        // other work): every workgroup has left and G is untouched
+        ++c.trd_timeouts;
+        ScopedTimer mark(c, "sytrd_timeout");  // counted fallback
+        lower = false;
+      }
+    }
+    if (!lower) {
+      if (rocsolver_dsytrd(h, rocblas_fill_upper, (rocblas_int)P, G, (rocblas_int)P, D, E,
+                           tau) != rocblas_status_success)
+        throw Error(SP_ERR_DEVICE, "rocsolver_dsytrd failed");
+      SP_HIP(hipMemcpyAsync(hd.data(), D, 2 * P * sizeof(double), hipMemcpyDeviceToHost,
+                            c.stream));
+      SP_HIP(hipStreamSynchronize(c.stream));
+    }
   }
-  std::vector<double> hd(2 * P), Z((u64)R * P);
-  SP_HIP(hipMemcpyAsync(hd.data(), D, 2 * P * sizeof(double), hipMemcpyDeviceToHost,
-                        c.stream));
-  SP_HIP(hipStreamSynchronize(c.stream));
   theta.resize(R);
   {
     ScopedTimer t(c, "tridiag_top");
@@ -740,8 +761,7 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
       SP_CHECK_LAUNCH();
     }
   }
-  // Z leaves host memory only after the copy has run
-  SP_HIP(hipStreamSynchronize(c.stream));
+  // V is queued on the stream; Z stays in the context's staging buffer
 }
 
 int scale_to_unit(Ctx &c, double *A, u64 count) {
@@ -761,13 +781,15 @@ int scale_to_unit(Ctx &c, double *A, u64 count) {
   return ex;
 }
 
-void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta) {
+void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta,
+             bool in_range) {
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds the matrix order");
   // A Gram far outside [2^-400, 2^400] (e.g. from a nearly vanishing Y) is
   // scaled by a power of two first (exact), as LAPACK's dsyevd scales its
   // input (rocSOLVER's does not), so that Householder sums of squares and
-  // pivots stay in range; the eigenvalues are scaled back.
-  const int ex = scale_to_unit(c, G, P * P);
+  // pivots stay in range; the eigenvalues are scaled back.  in_range: the
+  // caller has done so already.
+  const int ex = in_range ? 0 : scale_to_unit(c, G, P * P);
   if (c.eig_method >= 2) {
     eig_top_tridiag(c, G, P, R, V, theta);
     for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(theta[r], -ex);
@@ -970,7 +992,7 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   }
   if (!warm) {
     const auto t0 = std::chrono::steady_clock::now();
-    eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+    eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta, true);
     SP_HIP(hipStreamSynchronize(c.stream));
     c.direct_eig_ms = mean(c.direct_eig_ms, ms_since(t0));
   }
@@ -986,7 +1008,8 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   u32 good = 0;
   while (good < R && theta[good] > 0.0 && theta[good] > theta[0] * kGramRankTol) ++good;
   const double lmax = std::max(std::fabs(theta[0]), 1e-300);
-  SP_HIP(hipMemcpyAsync(theta_d, theta.data(), R * sizeof(double), hipMemcpyHostToDevice,
+  c.h_theta.assign(theta.begin(), theta.end());
+  SP_HIP(hipMemcpyAsync(theta_d, c.h_theta.data(), R * sizeof(double), hipMemcpyHostToDevice,
                         c.stream));
   DevBuf &M = c.ws_rp;
   M.reserve((u64)R * P * sizeof(double));
@@ -1000,7 +1023,6 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
     SP_CHECK_LAUNCH();
   }
   complete_basis(c, good);
-  SP_HIP(hipStreamSynchronize(c.stream));
 }
 
 void dense_eig_full(Ctx &c, double *G, u64 P, double *D, double *E, double *resid,

@@ -203,6 +203,12 @@ struct Ctx {
   bool warm_enabled = true;
   u32 warm_skip = 0;
   double warm_miss_ms = 0.0, direct_eig_ms = 0.0;
+  // host staging of the dense Y-side steps, kept across calls so that a
+  // host-to-device copy from them needs no wait before the function returns
+  // (the next write to one follows a synchronisation of the stream)
+  std::vector<double> h_eig_de, h_eig_z, h_theta, h_small;
+  int h_trd_abort = 0;
+  u64 gram_tiles_P = 0;  // P of the Gram tile list in ws_tiles (0: none)
   DevBuf ws_warm;  // its bases, products and small matrices
 
   // chunking of the work list for the current R: plan[0] with 64-owner
@@ -303,14 +309,19 @@ int scale_to_unit(Ctx &c, double *A, u64 count);
 // Top-R eigenpairs of the P x P symmetric matrix G (device, destroyed) by
 // rocSOLVER (c.eig_method): V (device, P x R row-major) and theta (host),
 // descending.
-void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta);
+void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta,
+             bool in_range = false);
 
 // Tridiagonal reduction of the symmetric P x P G (device, only read) in one
 // launch of co-resident workgroups (sytrd.hip): D, E, tau and the reflector
-// tails in Vr (P x P) in LAPACK's uplo = lower layout.  Returns false when P
-// does not fit the kernel's LDS budget or a wait between its workgroups
-// timed out.
-bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr);
+// tails in Vr (P x P) in LAPACK's uplo = lower layout, queued on the
+// context's stream.  Returns false, with nothing launched, when P does not
+// fit the kernel's LDS budget; otherwise *abort_flag points to a device word
+// that is nonzero once the kernel has run if a wait between its workgroups
+// timed out (then every workgroup left, G is untouched and D, E, tau, Vr are
+// not valid).
+bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr,
+                 const int **abort_flag);
 constexpr u64 kTrdMaxP = 2048;  // sytrd_lower returns false beyond this order
 
 // V = Q Z (V row-major P x R, Z column-major P x R, both device) for the Q of

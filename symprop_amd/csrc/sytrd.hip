@@ -396,7 +396,8 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
   return true;
 }
 
-bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr) {
+bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr,
+                 const int **abort_flag) {
   if (P < 2 || P > kTrdMaxP || P > (u64)kTrdThreads * kTrdMaxPer) return false;
   const int per = P <= (u64)kTrdThreads       ? 1
                   : P <= 2 * (u64)kTrdThreads ? 2
@@ -443,15 +444,8 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
   void *args[] = {&a};
   SP_HIP(hipLaunchKernel(kern, dim3(W), dim3(kTrdThreads), args, lds, c.stream));
   SP_CHECK_LAUNCH();
-  int h_abort = 0;
-  SP_HIP(hipMemcpyAsync(&h_abort, abort, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  SP_HIP(hipStreamSynchronize(c.stream));
-  if (h_abort) {
-    // some workgroup was not resident in time (the device is shared with other
-    // work): every workgroup has left and G is untouched; the caller falls back
-    ++c.trd_timeouts;
-    return false;
-  }
+  // the caller reads the flag with the kernel's output (one wait for both)
+  *abort_flag = abort;
   return true;
 }
 
