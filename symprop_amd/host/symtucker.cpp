@@ -135,14 +135,18 @@ private:
   bool _have_snapshot = false;
 };
 
+}  // namespace
+
 template <class F>
-void timed(const char *name, sp_ctx *ctx, F &&f) {
+void SpSymtensor::timed(const char *name, F &&f) const {
+  if (!_phase_timing) {
+    f();
+    return;
+  }
   FunctionTimer t(name);
   f();
-  ck(sp_sync(ctx));
+  ck(sp_sync(_ctx));
 }
-
-}  // namespace
 
 // ------------------------------------------------------------ packed helpers
 index_t symtens_size(index_t dim, index_t order) { return binom(order + dim - 1, order); }
@@ -381,10 +385,10 @@ SymTuckerDecomp SpSymtensor::TuckerHOOI(const Config &c) const {
   IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
   for (size_t iter = 0; iter < c.max_iter; iter++) {
-    timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
-    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
-    timed("leftsvd", _ctx, [&] { ck(sp_leftsvd_y(_ctx, nullptr)); });
-    timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
+    timed("regularize", [&] { ck(sp_regularize(_ctx, c.delta)); });
+    timed("S3TTMc", [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
+    timed("leftsvd", [&] { ck(sp_leftsvd_y(_ctx, nullptr)); });
+    timed("gemmtn", [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
     const real_t rel_res = symtens_rel_res(core_prev, core_p, coeff, R);
     const real_t core_norm = symtens_norm(core_p, coeff, R);
     const real_t error = fit_error(_norm, core_norm);  // reconstruct_error
@@ -421,11 +425,11 @@ SymTuckerDecomp SpSymtensor::TuckerHOQRI(const Config &c) const {
   IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
   for (size_t iter = 0; iter < c.max_iter; iter++) {
-    timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
-    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
-    timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core.data())); });
-    timed("symgemm", _ctx, [&] { ck(sp_symgemm(_ctx)); });
-    timed("qr", _ctx, [&] { ck(sp_qr(_ctx)); });
+    timed("regularize", [&] { ck(sp_regularize(_ctx, c.delta)); });
+    timed("S3TTMc", [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
+    timed("gemmtn", [&] { ck(sp_gemmtn(_ctx, core.data())); });
+    timed("symgemm", [&] { ck(sp_symgemm(_ctx)); });
+    timed("qr", [&] { ck(sp_qr(_ctx)); });
     const real_t rel_res = symtens_rel_res(core, core_prev, coeff, R);
     const real_t core_norm = symtens_norm(core, coeff, R);
     const real_t error = fit_error(_norm, core_norm);
@@ -468,9 +472,9 @@ SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
   IterationControl ctl(c, _ctx);
   real_t prev_error = 0.0;
   for (size_t iter = 0; iter < c.max_iter; iter++) {
-    timed("regularize", _ctx, [&] { ck(sp_regularize(_ctx, c.delta)); });
-    timed("S3TTMc", _ctx, [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
-    timed("syevd", _ctx, [&] {
+    timed("regularize", [&] { ck(sp_regularize(_ctx, c.delta)); });
+    timed("S3TTMc", [&] { ck(sp_s3ttmc_dev(_ctx, ctl.precision())); });
+    timed("syevd", [&] {
       ck(sp_leftsvd_y(_ctx, nullptr));
       ck(sp_factor_download(_ctx, factor.data()));
       for (size_t i = 0; i < _dim; ++i)
@@ -478,7 +482,7 @@ SymTuckerDecomp SpSymtensor::TuckerAltHOOI(const Config &c) const {
       ck(sp_factor_upload(_ctx, factor.data(), R));
     });
     const real_t rel_res = rel_residual(factor, factor_prev);
-    timed("gemmtn", _ctx, [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
+    timed("gemmtn", [&] { ck(sp_gemmtn(_ctx, core_p.data())); });
     std::copy(core_p.begin(), core_p.end(), core.begin());
     const real_t core_norm = symtens_norm(core_p, coeff, R);
     const real_t error = fit_error(_norm, core_norm);
@@ -527,6 +531,9 @@ extern "C" int sp_tucker_run(sp_ctx *ctx, const sp_tucker_opts *o, const double 
     if (o->precision < SP_PREC_FP64 || o->precision > SP_PREC_BF16)
       throw spamd::Error(SP_ERR_ARG, "unknown precision");
     SpSymtensor T(ctx);
+    // iteration times are recorded by the loop itself; no per-phase timers
+    // and no waits between the phases of an iteration
+    T.set_phase_timing(false);
     Config c;
     c.rank = o->rank;
     c.max_iter = o->max_iter;

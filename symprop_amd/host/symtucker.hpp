@@ -111,8 +111,15 @@ public:
 
   real_t norm() const { return _norm; }
   sp_ctx *context() const { return _ctx; }
+  // Per-phase FunctionTimer entries of the Tucker loops (regularize, S3TTMc,
+  // leftsvd, ...): each phase is timed to its end on the GPU, i.e. the host
+  // waits for the stream after it (default on, as the reference's timers
+  // measure each call).  Off: no timers and no waits between the phases.
+  void set_phase_timing(bool on) { _phase_timing = on; }
 
 private:
+  template <class F>
+  void timed(const char *name, F &&f) const;
   void init_factor(std::vector<real_t> &U, const Config &c) const;
   void hoevd_init(std::vector<real_t> &U, const Config &c) const;
 
@@ -122,6 +129,7 @@ private:
   real_t _norm = 0.0;
   sp_ctx *_ctx = nullptr;
   bool _owns = false;
+  bool _phase_timing = true;
 };
 
 // Packed (lexicographic sorted multi-index) helpers.
