@@ -803,6 +803,9 @@ def main():
                          "reference's opt5 over contiguous chunks of the workload, next to "
                          "the same iterations on the GPU, and exit")
     ap.add_argument("--chunk-nnz", type=int, default=600_000)
+    ap.add_argument("--hooi-only", type=int, default=0, metavar="ITERS",
+                    help="only run ITERS FP64 iterations of the workload's Tucker loop "
+                         "(after the tree build; used for kernel traces) and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(reference for HBM counter scaling) and exit")
@@ -838,6 +841,16 @@ def main():
     inds, vals = sp.generate(w["kind"], N, n, w["nnz"], w["param"], w["communities"], 0)
     nnz = len(vals)
     log("nnz", nnz)
+    if args.hooi_only:
+        tg = sp.SpSymtensor(n, N, inds, vals)
+        r = tg.tucker(alg=w.get("alg", 0), rank=R, max_iter=args.hooi_only, rel_tol=0.0,
+                      delta=w["delta"], seed=0)
+        print(json.dumps({"workload": args.workload, "iters": r["iters"],
+                          "seconds": r["seconds"],
+                          "iteration_times": [float(x) for x in np.diff(r["times"],
+                                                                        prepend=0.0)],
+                          "errors": [float(e) for e in r["errors"]]}), flush=True)
+        return
     if args.cpu_hooi_chunked:
         tg = sp.SpSymtensor(n, N, inds, vals)
         r = tg.tucker(alg=0, rank=R, max_iter=args.cpu_hooi_chunked, rel_tol=0.0,
