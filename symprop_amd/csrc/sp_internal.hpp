@@ -194,6 +194,9 @@ struct Ctx {
   // polls of one exchange wait before the reduction gives up (about a second;
   // SP_TRD_SPIN at context creation sets it, to test the fallback)
   unsigned trd_spin_limit = 1u << 21;
+  // P <= 256: the reduction in one workgroup (SP_SYTRD=wg at context
+  // creation keeps the multi-workgroup kernel for every P)
+  bool trd_one_wg = true;
   u64 trd_timeouts = 0;  // reductions that timed out and fell back
   // leftsvd's warm-started Rayleigh-Ritz eigen-step (dense.hip): on by default
   // (SP_EIG_WARM=0 at context creation turns it off).  After a miss the next

@@ -289,6 +289,361 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_kernel(TrdArgs a) {
   if ((P - 1) % W == wg && t == 0) a.d[P - 1] = x[P - 1];
 }
 
+// ---------------------------------------------------------------------------
+// One-workgroup form for P <= 256: the same arithmetic (dsytd2, uplo = 'L')
+// and output layout, with no exchange through global memory.
+//
+// The lower triangle of G, cut into 16 x 16 blocks (diagonal blocks stored
+// whole), lives in the registers of 512 threads: wave q holds list entries
+// q, q+8, q+16, ... of the blocks ordered by block column, last first, so the
+// blocks still active at column k (block column >= (k+1)/16) are a prefix of
+// every wave's list and the work stays balanced as the trailing block
+// shrinks.  In a block, lane l holds row l%16 and columns 4(l/16)..+3.
+//
+// Per column k, three phases and three workgroup barriers:
+//   A  every wave applies column k-1's rank-2 update to its blocks and forms
+//      the partial products of p = tau_k A v_k: block (I,J) gives its row
+//      sums to S[I][J] and, off the diagonal, its column sums to S[J][I];
+//      the holders of column k+1 copy it, as updated so far, to the LDS;
+//   B  wave q sums S[I][J] over J (fixed order) into p for the block rows
+//      I = Jk+q, Jk+q+8;
+//   C  wave 0 forms w = p - (tau/2)(p.v) v, applies the update to column
+//      k+1, derives the next reflector from it (dlarfg) and writes w, the new
+//      v and column k+1's d, e, tau to the LDS; four waves copy them (and
+//      v's tail) to the outputs during the next phase A.
+// The other waves wait at the third barrier: phase C is one dependent chain
+// (two wave sums, a square root, a division), and run by all waves at once it
+// would only take issue slots from each other.  Deterministic: every sum has
+// a fixed order, and the wave sums give every lane the same bits.
+//
+// Measured at P = 220 (cycles per column): phase A ~4,000, phase B ~770,
+// phase C ~2,000, barrier waits ~1,000; 0.69 ms against 0.83 ms for the
+// multi-workgroup form, 0.29 against 0.46 ms at P = 130.
+constexpr int kOwThreads = 512;
+constexpr int kOwWaves = kOwThreads / 64;
+constexpr int kOwNB = 16;  // block rows: P <= 256
+constexpr int kOwMaxP = 16 * kOwNB;
+constexpr int kOwBlocks = kOwNB * (kOwNB + 1) / 2;
+constexpr int kOwSlots = (kOwBlocks + kOwWaves - 1) / kOwWaves;  // blocks per wave
+constexpr int kOwSpare = kOwNB * kOwNB * 16;                    // spare row of S
+// LDS (doubles): S [NB][NB][16] + spare row, column k+1 [256] + spare row,
+// v [2][256], w [256], p [256], d/e/tau [2][3]
+constexpr size_t kOwLds = (size_t)(kOwSpare + 16 + 5 * kOwMaxP + 64 + 8) * sizeof(double);
+
+struct OwArgs {
+  const double *G;
+  double *Vr;
+  int P;
+  double *d, *e, *tau;
+};
+
+// Workgroup barrier ordering the LDS only (the global stores of the outputs
+// need no ordering inside the kernel).
+__device__ inline void ow_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// x summed with the same lane of the other half-row pair (lane ^ 16), in the
+// same order on both sides: v_permlane16_swap gives each lane of rows 0/1
+// (2/3) row 0's and row 1's (row 2's and row 3's) value.
+__device__ inline double ow_sum16(double x) {
+  const long long b = __double_as_longlong(x);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double a0 = __longlong_as_double(((long long)hi[0] << 32) | lo[0]);
+  const double a1 = __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+  return a0 + a1;
+}
+
+// x summed with lane ^ 32 (v_permlane32_swap: lower and upper half-wave).
+__device__ inline double ow_sum32(double x) {
+  const long long b = __double_as_longlong(x);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double a0 = __longlong_as_double(((long long)hi[0] << 32) | lo[0]);
+  const double a1 = __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+  return a0 + a1;
+}
+
+// One DPP move of a double within rows of 16 lanes (every source lane valid).
+template <int C>
+__device__ inline double ow_dpp(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, C, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), C, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the wave, the same bits in every lane: pairs that swap values
+// (lane ^ 1, ^ 2 in the quad, then the half-row and row mirrors, which pair
+// lanes whose partial sums are already equal within their quad / half-row,
+// then the two half-row and the two half-wave swaps) each add the same two
+// operands.
+__device__ inline double ow_wave_sum(double x) {
+  x += ow_dpp<0xb1>(x);
+  x += ow_dpp<0x4e>(x);
+  x += ow_dpp<0x141>(x);
+  x += ow_dpp<0x140>(x);
+  return ow_sum32(ow_sum16(x));
+}
+
+// Entry idx (uniform) of a vector held as x[u] at index lane + 64u.
+__device__ inline double ow_at(const double x[4], int idx) {
+  const int u = idx >> 6;
+  const double s = u == 0 ? x[0] : u == 1 ? x[1] : u == 2 ? x[2] : x[3];
+  return read_lane(s, idx & 63);
+}
+
+// dlarfg on x[kp:] (x at index lane + 64u, entries above kp ignored): v with
+// v[kp] = 1 and zeros elsewhere above, tau and beta.
+__device__ inline void ow_reflector(const double x[4], int kp, int P, int lane, double v[4],
+                                    double &tk, double &beta) {
+  double ss = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = lane + 64 * u;
+    if (j > kp && j < P) ss += x[u] * x[u];
+  }
+  ss = ow_wave_sum(ss);
+  const double alpha = ow_at(x, kp);
+  double scal = 0.0;
+  beta = alpha;
+  tk = 0.0;
+  if (ss > 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+    tk = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = lane + 64 * u;
+    v[u] = j == kp ? 1.0 : j > kp && j < P ? x[u] * scal : 0.0;
+  }
+}
+
+// Phase A on one block (I, J) held in a[4]: column k-1's update with vo, wo
+// (zero for k = 0), the row and column sums of a against vn into S, and
+// column k+1's entries into cbk.  Stores with nothing to store go to spare
+// LDS rows instead of branching (`on` false sends them all there).
+__device__ inline void ow_block(double a[4], int I, int J, bool on, int Jk, int kc, int lane,
+                                const double *vo, const double *wo, const double *vn,
+                                double *S, double *cbk) {
+  const int r = lane & 15, cg = lane >> 4;
+  const int i = I * 16 + r, j0 = J * 16 + 4 * cg;
+  const double voi = vo[i], woi = wo[i], vni = vn[i];
+  double voj[4], woj[4], vnj[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    voj[q] = vo[j0 + q];
+    woj[q] = wo[j0 + q];
+    vnj[q] = vn[j0 + q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] = fma(-woi, voj[q], fma(-voi, woj[q], a[q]));
+  // column k+1 (block column Jk, column kc of the block), as updated so far
+  const bool mine = on && J == Jk && cg == (kc >> 2);
+  const int qc = kc & 3;
+  cbk[mine ? i : kOwMaxP + lane] =
+      qc == 0 ? a[0] : qc == 1 ? a[1] : qc == 2 ? a[2] : a[3];
+  double rs = a[0] * vnj[0];
+#pragma unroll
+  for (int q = 1; q < 4; ++q) rs += a[q] * vnj[q];
+  rs = ow_sum32(ow_sum16(rs));  // the four column groups, same value in each
+  S[on ? (I * kOwNB + J) * 16 + r : kOwSpare + r] = rs;
+  // column sums over the block's 16 rows (the lanes of a DPP row): halves
+  // swapped with the mirrored lane (bit 3 differs), then with the half-row
+  // mirror (bit 2 differs), then summed over the quad; the quad then holds
+  // column 2*bit3 + bit2 of its group.  The diagonal block's rows already
+  // cover both halves.
+  const double c0 = a[0] * vni, c1 = a[1] * vni, c2 = a[2] * vni, c3 = a[3] * vni;
+  const bool h8 = lane & 8, h4 = lane & 4;
+  double k0 = h8 ? c2 : c0, k1 = h8 ? c3 : c1;
+  k0 += ow_dpp<0x140>(h8 ? c0 : c2);
+  k1 += ow_dpp<0x140>(h8 ? c1 : c3);
+  double y = h4 ? k1 : k0;
+  y += ow_dpp<0x141>(h4 ? k0 : k1);
+  y += ow_dpp<0xb1>(y);
+  y += ow_dpp<0x4e>(y);
+  S[on && I != J ? (J * kOwNB + I) * 16 + 4 * cg + (h8 ? 2 : 0) + (h4 ? 1 : 0) : kOwSpare + r] = y;
+}
+
+__global__ __launch_bounds__(kOwThreads) void sytrd_ow_kernel(OwArgs a) {
+  extern __shared__ double lds[];
+  double *S = lds;                   // [I][J][16] partial products of p, then a spare row
+  double *cb = S + kOwSpare + 16;    // column k+1 [256], then a spare row of 64
+  double *vb = cb + kOwMaxP + 64;    // [2][256] v, by parity of its column
+  double *wb = vb + 2 * kOwMaxP;     // [256] w
+  double *pb = wb + kOwMaxP;         // [256] p
+  double *sc = pb + kOwMaxP;         // [2][3] d, e, tau of a column, by parity
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int P = a.P, NB = (P + 15) >> 4, nblk = NB * (NB + 1) / 2;
+  const int r = lane & 15, cg = lane >> 4;
+
+  // this wave's blocks: list entry m = s*waves + wave lies on diagonal d of
+  // the list (block column NB-1-d) at offset o (block row NB-1-o); the
+  // coordinates (I << 4 | J) of four slots are packed per word
+  unsigned pk[(kOwSlots + 3) / 4] = {};
+  double A[kOwSlots][4];
+#pragma unroll
+  for (int s = 0; s < kOwSlots; ++s) {
+    const int m = s * kOwWaves + wave;
+    const bool on = m < nblk;
+    int d = 0;
+    while ((d + 1) * (d + 2) / 2 <= m) ++d;
+    const int I = on ? NB - 1 - (m - d * (d + 1) / 2) : 0, J = on ? NB - 1 - d : 0;
+    pk[s / 4] |= (unsigned)(I << 4 | J) << (8 * (s % 4));
+    const int i = I * 16 + r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = J * 16 + 4 * cg + q;
+      // lower triangle, column-major (its transpose above the diagonal)
+      A[s][q] = on && i < P && j < P
+                    ? (i >= j ? a.G[(size_t)j * P + i] : a.G[(size_t)i * P + j])
+                    : 0.0;
+    }
+  }
+
+  // wave 0's state: v of the current column and its tau
+  double vk[4] = {0.0, 0.0, 0.0, 0.0}, tk = 0.0;
+  if (wave == 0) {
+    double x[4], beta;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = lane + 64 * u;
+      x[u] = j < P ? a.G[j] : 0.0;
+    }
+    ow_reflector(x, 1, P, lane, vk, tk, beta);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = lane + 64 * u;
+      vb[j] = vk[u];
+      vb[kOwMaxP + j] = 0.0;
+      wb[j] = 0.0;
+      if (j >= 2 && j < P) a.Vr[j] = vk[u];
+    }
+    if (lane == 0) {
+      a.d[0] = x[0];
+      a.e[0] = beta;
+      a.tau[0] = tk;
+    }
+  }
+  ow_barrier();
+
+  for (int k = 0; k + 1 < P; ++k) {
+    const int par = k & 1;
+    const int Jk = (k + 1) >> 4, kc = (k + 1) & 15;
+    const int na = (NB - Jk) * (NB - Jk + 1) / 2;  // active blocks
+    const double *vo = vb + (par ^ 1) * kOwMaxP;   // v of column k-1 (zero for k = 0)
+    const double *vn = vb + par * kOwMaxP;         // v of column k
+    if (k > 0 && wave < 4) {
+      // column k's outputs (phase C of column k-1 left them in the LDS), a
+      // quarter of v's tail per wave
+      const int j = lane + 64 * wave;
+      if (j > k + 1 && j < P) a.Vr[(size_t)k * P + j] = vn[j];
+      if (wave == 0 && lane == 0) {
+        const double *o = sc + 3 * par;
+        a.d[k] = o[0];
+        a.e[k] = o[1];
+        a.tau[k] = o[2];
+      }
+    }
+    // phase A
+#pragma unroll
+    for (int s = 0; s < kOwSlots; ++s) {
+      if (s * kOwWaves + wave >= na) break;
+      const unsigned c = (pk[s / 4] >> (8 * (s % 4))) & 0xff;
+      ow_block(A[s], (int)(c >> 4), (int)(c & 15), true, Jk, kc, lane, vo, wb, vn, S, cb);
+    }
+    ow_barrier();
+    // phase B: the partial products of block rows Jk+wave, Jk+wave+8, four
+    // lanes per row (J = Jk+cg, Jk+cg+4, ...), then summed over the four
+    {
+      double acc[2] = {0.0, 0.0};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int I = Jk + wave + kOwWaves * h;
+        if (I < NB)
+          for (int J = Jk + cg; J < NB; J += 4) acc[h] += S[(I * kOwNB + J) * 16 + r];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int I = Jk + wave + kOwWaves * h;
+        const double x = ow_sum32(ow_sum16(acc[h]));
+        if (I < NB && cg == 0) pb[I * 16 + r] = x;
+      }
+    }
+    ow_barrier();
+    // phase C (wave 0); v of column k is 1 at k+1, so w and the updated
+    // column there and at k+2 follow from entries every lane reads
+    if (wave == 0) {
+      const bool more = k + 2 < P;  // column k+1 has a reflector
+      const double pk1 = tk * pb[k + 1], ck1 = cb[k + 1];
+      const double pk2 = more ? tk * pb[k + 2] : 0.0, ck2 = more ? cb[k + 2] : 0.0;
+      const double vk2 = more ? vn[k + 2] : 0.0;
+      double p[4], xc[4], w[4], pv = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = lane + 64 * u;
+        const bool live = j > k && j < P;
+        p[u] = live ? tk * pb[j] : 0.0;
+        xc[u] = live ? cb[j] : 0.0;
+        pv += p[u] * vk[u];
+      }
+      const double al = -0.5 * tk * ow_wave_sum(pv);
+      const double wk1 = pk1 + al;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = lane + 64 * u;
+        w[u] = p[u] + al * vk[u];
+        xc[u] = j > k && j < P ? xc[u] - (vk[u] * wk1 + w[u]) : 0.0;
+        wb[j] = w[u];
+      }
+      double vx[4] = {0.0, 0.0, 0.0, 0.0}, tn = 0.0, bn = 0.0;
+      if (more) {
+        // dlarfg on the updated column k+1 below k+1
+        const double alpha = ck2 - (vk2 * wk1 + (pk2 + al * vk2));
+        double ss = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = lane + 64 * u;
+          if (j > k + 2) ss += xc[u] * xc[u];
+        }
+        ss = ow_wave_sum(ss);
+        double scal = 0.0;
+        bn = alpha;
+        if (ss > 0.0) {
+          bn = -copysign(sqrt(alpha * alpha + ss), alpha);
+          tn = (bn - alpha) / bn;
+          scal = 1.0 / (alpha - bn);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = lane + 64 * u;
+          vx[u] = j == k + 2 ? 1.0 : j > k + 2 ? xc[u] * scal : 0.0;
+          vb[(par ^ 1) * kOwMaxP + j] = vx[u];
+        }
+      }
+      // column k+1's d, e, tau for the writer of the outputs (d alone for the last)
+      if (lane == 0) {
+        double *o = sc + 3 * (par ^ 1);
+        o[0] = ck1 - (wk1 + wk1);
+        o[1] = bn;
+        o[2] = tn;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) vk[u] = vx[u];
+      tk = tn;
+    }
+    ow_barrier();
+  }
+  if (wave == 0 && lane == 0) a.d[P - 1] = sc[3 * ((P - 1) & 1)];
+}
+
 // C = Q Z for Q = H_0 H_1 ... H_{P-2} of sytrd_lower (LAPACK dormtr, side
 // left, uplo lower, no transpose).  Each wave owns one column of Z, held in
 // registers (lane l has rows l + 64m, m < M); the reflectors are applied from
@@ -399,6 +754,18 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
 bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr,
                  const int **abort_flag) {
   if (P < 2 || P > kTrdMaxP || P > (u64)kTrdThreads * kTrdMaxPer) return false;
+  if (P <= (u64)kOwMaxP && c.trd_one_wg) {
+    // one workgroup, no waits between workgroups: the flag stays zero
+    c.ws_trd.reserve(sizeof(int));
+    int *abort = c.ws_trd.as<int>();
+    SP_HIP(hipMemsetAsync(abort, 0, sizeof(int), c.stream));
+    SP_HIP(hipFuncSetAttribute((const void *)sytrd_ow_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOwLds));
+    sytrd_ow_kernel<<<1, kOwThreads, kOwLds, c.stream>>>(OwArgs{G, Vr, (int)P, D, E, tau});
+    SP_CHECK_LAUNCH();
+    *abort_flag = abort;
+    return true;
+  }
   const int per = P <= (u64)kTrdThreads       ? 1
                   : P <= 2 * (u64)kTrdThreads ? 2
                   : P <= 4 * (u64)kTrdThreads ? 4
