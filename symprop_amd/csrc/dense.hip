@@ -945,6 +945,57 @@ double eig_top_warm(Ctx &c, const double *G, u64 P, u32 R, double *V,
   return worst;
 }
 
+// Top-R eigenpairs of the Gram G (P <= 256, in range after scale_to_unit) by
+// Rayleigh-Ritz on the m-block Krylov space of X0 = Z^T U (the warm start of
+// eig_top_warm): krylov_basis builds orthonormal rows Q and Q G in one
+// workgroup, H = Q G Q^T (m R x m R) goes through the direct solver (eig_top),
+// and the Ritz vectors Y = S^T Q come with their residuals ||G y - theta y||.
+// Returns the largest residual / theta_1 (infinity when theta_R is not above
+// 1e-8 theta_1 or the basis could not be built); V and theta as
+// eig_top_warm's.
+double eig_top_krylov(Ctx &c, const double *G, u64 P, u32 R, u32 m, double *V,
+                      std::vector<double> &theta) {
+  constexpr double kFail = std::numeric_limits<double>::infinity();
+  const u64 n = c.dim, RP = (u64)R * P, k = (u64)m * R;
+  // Xt, Yt, GYt: [R][P]; Qt, GQt: [k][P]; H: [k][k]; S: [k][R]; theta, residuals: R
+  c.ws_warm.reserve((3 * RP + 2 * k * P + k * k + k * R + 2 * R) * sizeof(double));
+  double *Xt = c.ws_warm.as<double>();
+  double *Yt = Xt + RP;
+  double *GYt = Yt + RP;
+  double *Qt = GYt + RP;
+  double *GQt = Qt + k * P;
+  double *H = GQt + k * P;
+  double *S = H + k * k;
+  double *th_d = S + k * R;
+  double *res_d = th_d + R;
+  atb(c, c.d_U.as<double>(), R, c.d_Y.as<double>(), P, n, Yt, "warm_x0");  // U^T Y
+  scale_cols_sqrt<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, c.d_W.as<double>(), R, P, Xt);
+  SP_CHECK_LAUNCH();
+  if (!krylov_basis(c, G, Xt, P, R, m, Qt, GQt)) return kFail;
+  y_times_mt<<<ymt_grid(k, k), 256, 0, c.stream>>>(Qt, k, P, GQt, k, H);
+  SP_CHECK_LAUNCH();
+  std::vector<double> th;
+  eig_top(c, H, k, R, S, th, true);
+  if (!(th[0] > 0.0) || !std::isfinite(th[0]) || !(th[R - 1] > th[0] * 1e-8)) return kFail;
+  atb(c, S, R, Qt, P, k, Yt, "warm_atb");
+  atb(c, S, R, GQt, P, k, GYt, "warm_atb");
+  c.h_small.assign(th.begin(), th.end());
+  SP_HIP(hipMemcpyAsync(th_d, c.h_small.data(), R * sizeof(double), hipMemcpyHostToDevice,
+                        c.stream));
+  resid_sq<<<R, 256, 0, c.stream>>>(GYt, Yt, th_d, P, res_d);
+  SP_CHECK_LAUNCH();
+  std::vector<double> res(R);
+  SP_HIP(hipMemcpyAsync(res.data(), res_d, R * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+  SP_HIP(hipStreamSynchronize(c.stream));
+  double worst = 0.0;
+  for (u32 r = 0; r < R; ++r) worst = std::max(worst, std::sqrt(res[r]));
+  transpose_nr<<<grid_for(RP, 256), 256, 0, c.stream>>>(Yt, R, P, V);
+  SP_CHECK_LAUNCH();
+  theta = th;
+  return worst / th[0];
+}
+
 void dense_leftsvd_y(Ctx &c, double *sigma) {
   dense_gram(c);
   const u64 P = c.P, n = c.dim;
@@ -969,9 +1020,20 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
     } else {
       double res;
       const auto t0 = std::chrono::steady_clock::now();
+      // P <= 256: the block Krylov form (its own blocks, m, adapt: one fewer
+      // after a result two digits inside the bar, two more after a miss);
+      // above, the one- or two-block step
+      const u32 mmax = (u32)std::min<u64>(12, P / R);
+      const bool kry = P <= 256 && R <= 16 && mmax >= 2;
+      if (kry && c.kry_m == 0) c.kry_m = std::min<u32>(8, mmax);
       {
         ScopedTimer t(c, "eig_warm");
-        res = eig_top_warm(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+        res = kry ? eig_top_krylov(c, c.d_G.as<double>(), P, R, c.kry_m, V.as<double>(), theta)
+                  : eig_top_warm(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
+      }
+      if (kry) {
+        if (res <= 1e-15 && c.kry_m > 2) --c.kry_m;
+        else if (res > 1e-13) c.kry_m = std::min(mmax, c.kry_m + 2);
       }
       // kept at the direct solver's accuracy.  A near miss (the loop is
       // closing in on its fixed point) is tried again at the next call;

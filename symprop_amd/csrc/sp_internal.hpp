@@ -205,6 +205,7 @@ struct Ctx {
   // direct path's time; a near miss or a success clears it
   bool warm_enabled = true;
   u32 warm_skip = 0;
+  u32 kry_m = 0;  // blocks of the Krylov form for P <= 256 (0: not chosen yet)
   double warm_miss_ms = 0.0, direct_eig_ms = 0.0;
   // host staging of the dense Y-side steps, kept across calls so that a
   // host-to-device copy from them needs no wait before the function returns
@@ -326,6 +327,14 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
 bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr,
                  const int **abort_flag);
 constexpr u64 kTrdMaxP = 2048;  // sytrd_lower returns false beyond this order
+
+// Orthonormal rows Qt (m b x P) of the block Krylov space of the b rows X0t
+// (b x P) under the symmetric G (P x P), and GQt = Qt G, in one workgroup
+// (eig_krylov.hip); rows past a breakdown (an invariant subspace) are zero.
+// Queued on the context's stream; false, with nothing launched, unless
+// P <= 256, b <= 16 and m b <= P.
+bool krylov_basis(Ctx &c, const double *G, const double *X0t, u64 P, u32 b, u32 m, double *Qt,
+                  double *GQt);
 
 // V = Q Z (V row-major P x R, Z column-major P x R, both device) for the Q of
 // sytrd_lower's output in Vr (passed as G) and tau (sytrd.hip); queued on the

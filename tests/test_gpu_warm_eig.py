@@ -102,3 +102,29 @@ def test_warm_leftsvd_matches_numpy_svd():
     assert np.allclose(Un.T @ Un, np.eye(R), atol=1e-12)
     c = np.linalg.svd(Un.T @ Us[:, :R], compute_uv=False)
     assert np.allclose(c, 1.0, atol=1e-10)
+
+
+def test_krylov_step_on_zipf_order4(monkeypatch):
+    # P = 220 (order 4, rank 10): leftsvd's block Krylov form.  On this kind of
+    # tensor the loop's fixed point is not an exact singular subspace of Y
+    # (row clipping), so the one-block step never applies; the Krylov space
+    # of X0 reaches the direct solver's accuracy after a few iterations, and
+    # the trajectory must not depend on which of the two gave an iteration
+    N, n, R = 4, 2000, 10
+    inds, vals = sp.generate(sp.GEN_ZIPF, N, n, 100_000, 1.1, 1, seed=2)
+    kw = dict(alg=0, rank=R, max_iter=30, rel_tol=0.0, delta=0.5, seed=0)
+    say("krylov eigen-step, zipf order 4, direct")
+    monkeypatch.setenv("SP_EIG_WARM", "0")
+    td = sp.SpSymtensor(n, N, inds, vals)
+    rd = td.tucker(**kw)
+    say("krylov eigen-step, zipf order 4, krylov")
+    monkeypatch.delenv("SP_EIG_WARM")
+    tw = sp.SpSymtensor(n, N, inds, vals)
+    tw.timing(True)
+    rw = tw.tucker(**kw)
+    ok = tw.timing_get("eig_warm_ok")[1]
+    say("krylov steps taken", ok, "of", tw.timing_get("eig_warm")[1], "tried")
+    assert ok >= 15
+    assert np.allclose(rw["errors"], rd["errors"], rtol=0, atol=1e-12)
+    assert np.allclose(rw["core_norms"], rd["core_norms"], rtol=1e-12, atol=0)
+    assert tucker.subspace_cos(rw["U"], rd["U"]) > 1 - 1e-12
