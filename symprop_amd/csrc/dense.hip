@@ -1020,31 +1020,42 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
     } else {
       double res;
       const auto t0 = std::chrono::steady_clock::now();
-      // P <= 256: the block Krylov form (its own blocks, m, adapt: one fewer
-      // after a result two digits inside the bar, two more after a miss);
-      // above, the one- or two-block step
+      // P <= 256: the block Krylov form; above, the one- or two-block step.
+      // Its number of blocks m adapts: two more after a miss, and that m+1 is
+      // then the floor for the next 64 calls; one fewer after a residual two
+      // digits inside the bar while above the floor
       const u32 mmax = (u32)std::min<u64>(12, P / R);
       const bool kry = P <= 256 && R <= 16 && mmax >= 2;
       if (kry && c.kry_m == 0) c.kry_m = std::min<u32>(8, mmax);
+      if (kry && c.kry_m > mmax) c.kry_m = mmax;
+      const bool widest = kry && c.kry_m >= mmax;
       {
         ScopedTimer t(c, "eig_warm");
         res = kry ? eig_top_krylov(c, c.d_G.as<double>(), P, R, c.kry_m, V.as<double>(), theta)
                   : eig_top_warm(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
       }
       if (kry) {
-        if (res <= 1e-15 && c.kry_m > 2) --c.kry_m;
-        else if (res > 1e-13) c.kry_m = std::min(mmax, c.kry_m + 2);
+        if (res > 1e-13) {
+          c.kry_floor = std::min(mmax, c.kry_m + 1);
+          c.kry_hold = 64;
+          c.kry_m = std::min(mmax, c.kry_m + 2);
+        } else {
+          if (c.kry_hold > 0 && --c.kry_hold == 0) c.kry_floor = 2;
+          if (res <= 1e-15 && c.kry_m > std::max<u32>(2, c.kry_floor)) --c.kry_m;
+        }
       }
       // kept at the direct solver's accuracy.  A near miss (the loop is
       // closing in on its fixed point) is tried again at the next call;
-      // after a clear miss the next calls skip the attempt, as many as keep
-      // the misses within a twentieth of the direct path's time
+      // after a clear miss (or a miss of the Krylov form at its most blocks,
+      // where its own rounding can keep the residual just above the bar) the
+      // next calls skip the attempt, as many as keep the misses within a
+      // twentieth of the direct path's time
       warm = res <= 1e-13;
       if (warm) {
         ScopedTimer mark(c, "eig_warm_ok");  // counted success
       } else {
         c.warm_miss_ms = mean(c.warm_miss_ms, ms_since(t0));
-        if (res > 1e-6)
+        if (res > 1e-6 || widest)
           c.warm_skip = c.direct_eig_ms > 0.0
                             ? (u32)std::min(64.0, std::ceil(c.warm_miss_ms /
                                                             (0.05 * c.direct_eig_ms)) - 1.0)

@@ -7,8 +7,11 @@
 // Krylov space span{X0, G X0, ..., G^(m-1) X0} of the b starting rows X0
 // (the warm start Z^T U of leftsvd) and the rows of Q G:
 //   Q_0 = CholQR2(X0);  for j = 0..m-1:  (Q G)_j = Q_j G,  and for j < m-1
-//   Q_{j+1} = CholQR2(Z) with Z = (Q G)_j projected off Q_0..Q_j twice
-//   (classical Gram-Schmidt, twice: orthogonal to working precision).
+//   Q_{j+1} = CholQR(proj(CholQR(proj(Z)))) with Z = (Q G)_j and proj the
+//   classical Gram-Schmidt projection off Q_0..Q_j: the second projection
+//   follows the first normalisation, so the block is orthogonal to the
+//   earlier ones to working precision even where the first normalisation
+//   scaled up a direction of small norm.
 // A block whose projected norm falls below 1e-10 of its norm before the
 // projection means the space is invariant to working precision: the basis
 // stops there and the remaining rows of Q and Q G are zero (they add zero
@@ -352,11 +355,14 @@ __global__ __launch_bounds__(kKyThreads) void krylov_basis_kernel(KyArgs a) {
     ky_gprod<B>(a, j * b, Zb, QC);
     built = j + 1;
     if (j + 1 == m) break;
+    // project, normalise, and again: the second projection comes after the
+    // first normalisation, which scales up whatever of the earlier blocks the
+    // first left in directions of small norm
     ky_row_norms<B>(a, Zb, ref);
-    ky_project<B>(a, (j + 1) * b, Zb, QC);
     ky_project<B>(a, (j + 1) * b, Zb, QC);
     ok = ky_cholqr<B>(a, Zb, Mg, part, ref, flag);
     if (ok) {
+      ky_project<B>(a, (j + 1) * b, Zb, QC);
       for (int x = t; x < b; x += kKyThreads) ref[x] = 1.0;
       ky_barrier();
       ok = ky_cholqr<B>(a, Zb, Mg, part, ref, flag);

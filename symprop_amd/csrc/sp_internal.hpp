@@ -206,6 +206,7 @@ struct Ctx {
   bool warm_enabled = true;
   u32 warm_skip = 0;
   u32 kry_m = 0;  // blocks of the Krylov form for P <= 256 (0: not chosen yet)
+  u32 kry_floor = 2, kry_hold = 0;  // least m while kry_hold calls remain
   double warm_miss_ms = 0.0, direct_eig_ms = 0.0;
   // host staging of the dense Y-side steps, kept across calls so that a
   // host-to-device copy from them needs no wait before the function returns
