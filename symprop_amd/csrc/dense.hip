@@ -132,6 +132,30 @@ __global__ void scale_by(double *__restrict__ a, u64 n, double s) {
   if (i < n) a[i] *= s;
 }
 
+// The largest of the nb partial maxima (fixed-order tree), and the power of
+// two that brings it into [1, 2) when it lies outside [2^-400, 2^400]
+// (finite, nonzero; 0 otherwise) into *ex.  One block of 256 threads.
+__global__ void __launch_bounds__(256) scale_pick(const double *__restrict__ part, int nb,
+                                                  int *__restrict__ ex) {
+  __shared__ double m[256];
+  m[threadIdx.x] = (int)threadIdx.x < nb ? part[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = m[0];
+    *ex = a > 0.0 && isfinite(a) && (a < 0x1p-400 || a > 0x1p400) ? -ilogb(a) : 0;
+  }
+}
+
+__global__ void scale_by_pow2(double *__restrict__ a, u64 n, const int *__restrict__ ex) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = *ex;
+  if (i < n && e != 0) a[i] = ldexp(a[i], e);
+}
+
 // The same sums for short outputs (a few hundred entries, where one thread
 // per entry would add its nb partials one after another): one wavefront per
 // entry, lanes over the partials, then a fixed-order wave sum.
@@ -781,6 +805,20 @@ int scale_to_unit(Ctx &c, double *A, u64 count) {
   return ex;
 }
 
+void scale_to_unit_async(Ctx &c, double *A, u64 count) {
+  constexpr unsigned kBlocks = 256;
+  c.ws_amax.reserve(kBlocks * sizeof(double) + sizeof(int));
+  double *part = c.ws_amax.as<double>();
+  int *ex = reinterpret_cast<int *>(part + kBlocks);
+  absmax_partial<<<kBlocks, 256, 0, c.stream>>>(A, count, part);
+  SP_CHECK_LAUNCH();
+  scale_pick<<<1, 256, 0, c.stream>>>(part, (int)kBlocks, ex);
+  SP_CHECK_LAUNCH();
+  scale_by_pow2<<<grid_for(count, 256), 256, 0, c.stream>>>(A, count, ex);
+  SP_CHECK_LAUNCH();
+  SP_HIP(hipMemcpyAsync(c.h_pin.get(), ex, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+}
+
 void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &theta,
              bool in_range) {
   if (R > P) throw Error(SP_ERR_ARG, "rank exceeds the matrix order");
@@ -1007,7 +1045,9 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   DevBuf &V = c.ws_pr;
   V.reserve(P * R * sizeof(double));
   std::vector<double> theta;
-  const int ex = scale_to_unit(c, c.d_G.as<double>(), P * P);
+  // G scaled into range on the device; the exponent is read after the
+  // eigen-step below, every path of which synchronises the stream
+  scale_to_unit_async(c, c.d_G.as<double>(), P * P);
   bool warm = false;
   auto ms_since = [](std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
@@ -1069,6 +1109,7 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
     SP_HIP(hipStreamSynchronize(c.stream));
     c.direct_eig_ms = mean(c.direct_eig_ms, ms_since(t0));
   }
+  const int ex = c.h_pin.get()[0];
   for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(theta[r], -ex);
   c.ws_eig.reserve((2 * P + 2 + R) * sizeof(double) + 4 * sizeof(rocblas_int));
   double *theta_d = c.ws_eig.as<double>();

@@ -126,6 +126,23 @@ struct Timer {
   hipEvent_t a, b;
 };
 
+// A few words of pinned host memory (hipHostMalloc), for device-to-host
+// copies that must not wait for the stream: the host reads them after a
+// later synchronisation.
+struct PinWords {
+  int *p = nullptr;
+  PinWords() = default;
+  PinWords(const PinWords &) = delete;
+  PinWords &operator=(const PinWords &) = delete;
+  ~PinWords() {
+    if (p) (void)hipHostFree(p);
+  }
+  int *get() {
+    if (!p) SP_HIP(hipHostMalloc((void **)&p, 16 * sizeof(int), hipHostMallocDefault));
+    return p;
+  }
+};
+
 struct Ctx {
   u32 order = 0, dim = 0;
   u64 nnz = 0;
@@ -190,7 +207,8 @@ struct Ctx {
   DevBuf ws_sym;   // symmetric matrix uploaded by sp_syevd_top
   DevBuf ws_trd;   // tridiagonal reduction: broadcast vectors, barrier words
   DevBuf ws_trd_v; // tridiagonal reduction: reflector tails (P x P)
-  DevBuf ws_amax;  // eigen-step: per-block maxima of |G|
+  DevBuf ws_amax;  // eigen-step: per-block maxima of |G|, then the scaling exponent
+  PinWords h_pin;  // [0]: leftsvd's scaling exponent (scale_to_unit_async)
   // polls of one exchange wait before the reduction gives up (about a second;
   // SP_TRD_SPIN at context creation sets it, to test the fallback)
   unsigned trd_spin_limit = 1u << 21;
@@ -310,6 +328,10 @@ void hoevd_init(Ctx &c, u32 R, double *U_out, double *eig_out);
 // lies in [1, 2) when that largest value is outside [2^-400, 2^400] (finite,
 // nonzero); returns ex (0: not scaled).
 int scale_to_unit(Ctx &c, double *A, u64 count);
+// The same decision and scaling queued on the stream without a wait: the
+// exponent lands in c.h_pin.get()[0] once the stream has passed this point
+// (read it only after a later synchronisation of the stream).
+void scale_to_unit_async(Ctx &c, double *A, u64 count);
 
 // Top-R eigenpairs of the P x P symmetric matrix G (device, destroyed) by
 // rocSOLVER (c.eig_method): V (device, P x R row-major) and theta (host),
