@@ -87,7 +87,8 @@ int sp_ctx_create(uint32_t order, uint32_t dim, uint64_t nnz, const uint32_t *in
       if (const char *e = std::getenv("SP_TRD_SPIN"))
         c.trd_spin_limit = (unsigned)std::max(1L, std::strtol(e, nullptr, 10));
       if (const char *e = std::getenv("SP_EIG_WARM")) c.warm_enabled = std::strcmp(e, "0") != 0;
-      if (const char *e = std::getenv("SP_SYTRD")) c.trd_one_wg = std::strcmp(e, "wg") != 0;
+      if (const char *e = std::getenv("SP_SYTRD"))
+        c.trd_mode = std::strcmp(e, "wg") == 0 ? 1 : std::strcmp(e, "reg") == 0 ? 2 : 0;
       if (const char *e = std::getenv("SP_EIG"))
         c.eig_method = std::strcmp(e, "syevd") == 0               ? 0
                        : std::strcmp(e, "syevj") == 0              ? 1

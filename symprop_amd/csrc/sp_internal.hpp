@@ -212,9 +212,11 @@ struct Ctx {
   // polls of one exchange wait before the reduction gives up (about a second;
   // SP_TRD_SPIN at context creation sets it, to test the fallback)
   unsigned trd_spin_limit = 1u << 21;
-  // P <= 256: the reduction in one workgroup (SP_SYTRD=wg at context
-  // creation keeps the multi-workgroup kernel for every P)
-  bool trd_one_wg = true;
+  // tridiagonal reduction: 0 (default) one workgroup for P <= 256, rows in
+  // LDS across workgroups up to P = 1,024, rows in registers across
+  // workgroups above; 1 (SP_SYTRD=wg at context creation) rows in LDS for
+  // every P; 2 (SP_SYTRD=reg) rows in registers for every P
+  int trd_mode = 0;
   u64 trd_timeouts = 0;  // reductions that timed out and fell back
   // leftsvd's warm-started Rayleigh-Ritz eigen-step (dense.hip): on by default
   // (SP_EIG_WARM=0 at context creation turns it off).  After a miss the next

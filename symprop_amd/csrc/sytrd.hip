@@ -644,6 +644,186 @@ __global__ __launch_bounds__(kOwThreads) void sytrd_ow_kernel(OwArgs a) {
   if (wave == 0 && lane == 0) a.d[P - 1] = sc[3 * ((P - 1) & 1)];
 }
 
+// ---------------------------------------------------------------------------
+// Multi-workgroup form with the rows in registers (P > 1,024).  The same
+// exchange as sytrd_lower_kernel, but each workgroup's rows (at most 16:
+// rows wg, wg+W, ...) live in the registers of its 512 threads, thread t
+// holding columns t, t+512, ... of every row, so the symv and the rank-2
+// update read no memory but the vectors.  A row's sum over the workgroup:
+// the 16 rows' partial sums reduced over each wave at once (half-wave and
+// half-row swaps, DPP mirrors, quad swaps: 16 values in about 60 VALU
+// instructions, lane l ending with row (l/4) mod 16), then over the waves
+// through the LDS in a fixed order.
+constexpr int kRgRows = 16;
+
+// (a' + b') of v_permlane32_swap(a, b) / v_permlane16_swap(a, b): the lanes
+// of the lower half-wave (even half-rows) get their a plus the upper (odd)
+// partner's a, the others their b plus the partner's b.
+__device__ inline double rg_pair32(double a, double b) {
+  const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)y, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane32_swap((unsigned)(x >> 32), (unsigned)(y >> 32), false, false);
+  return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
+         __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+__device__ inline double rg_pair16(double a, double b) {
+  const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)y, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane16_swap((unsigned)(x >> 32), (unsigned)(y >> 32), false, false);
+  return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
+         __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+
+// x[q] summed over the wave; lane l returns the sum of x[(l >> 2) & 15] (the
+// four lanes of a quad the same bits).
+__device__ inline double rg_reduce16(const double x[16], int lane) {
+  double y[8], z[4], w[2];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) y[q] = rg_pair32(x[q], x[q + 8]);  // value q + 8*bit5
+#pragma unroll
+  for (int q = 0; q < 4; ++q) z[q] = rg_pair16(y[q], y[q + 4]);  // value q + 4*bit4 + 8*bit5
+  const bool h8 = lane & 8, h4 = lane & 4;
+#pragma unroll
+  for (int q = 0; q < 2; ++q)  // bit 3: row mirror (lane ^ 15)
+    w[q] = (h8 ? z[q + 2] : z[q]) + ow_dpp<0x140>(h8 ? z[q] : z[q + 2]);
+  double u = (h4 ? w[1] : w[0]) + ow_dpp<0x141>(h4 ? w[0] : w[1]);  // bit 2: half-row mirror
+  u += ow_dpp<0x4e>(u);
+  u += ow_dpp<0xb1>(u);
+  return u;
+}
+
+template <int PER>
+__global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
+  extern __shared__ double lds[];
+  __shared__ double red[kTrdWaves];
+  __shared__ int fail;
+  const int P = a.P, W = a.W, wg = blockIdx.x, t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  double *v = lds;                    // [P] reflector of the column
+  double *w = v + P;                  // [P]
+  double *x = w + P;                  // [P] pivot row
+  double *rs = x + P;                 // [16][waves] each wave's sums of the local rows
+  double *pvs = rs + kRgRows * kTrdWaves;  // [16] p_i v_i of the local rows
+  const int nrows = wg < P ? (P - wg + W - 1) / W : 0;
+  double A[kRgRows][PER];
+#pragma unroll
+  for (int s = 0; s < kRgRows; ++s)
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = t + kTrdThreads * u;
+      A[s][u] = s < nrows && j < P ? a.G[(size_t)(wg + s * W) * P + j] : 0.0;
+    }
+  for (int j = t; j < P; j += kTrdThreads) x[j] = a.G[j];
+  __syncthreads();
+  for (int k = 0; k + 1 < P; ++k) {
+    const int par = k & 1;
+    double tk, beta;
+    reflector(x, k, P, v, red, tk, beta);
+    // this thread's columns of v (v is stale at and before k)
+    double rv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = t + kTrdThreads * u;
+      rv[u] = j > k && j < P ? v[j] : 0.0;
+    }
+    {
+      double ps[kRgRows];
+#pragma unroll
+      for (int s = 0; s < kRgRows; ++s) {
+        double acc = A[s][0] * rv[0];
+#pragma unroll
+        for (int u = 1; u < PER; ++u) acc += A[s][u] * rv[u];
+        ps[s] = acc;
+      }
+      const double r = rg_reduce16(ps, lane);
+      if ((lane & 3) == 0) rs[((lane >> 2) & 15) * kTrdWaves + wave] = r;
+    }
+    double *xb = a.xbuf + (size_t)par * P;
+    if ((k + 1) % W == wg) {  // row k+1 as it stands, for every workgroup
+      const int s1 = (k + 1) / W;
+#pragma unroll
+      for (int s = 0; s < kRgRows; ++s)
+        if (s == s1) {
+#pragma unroll
+          for (int u = 0; u < PER; ++u) {
+            const int j = t + kTrdThreads * u;
+            if (j > k && j < P) xb[j] = A[s][u];
+          }
+        }
+    }
+    __syncthreads();
+    double *pb = a.pbuf + (size_t)par * P;
+    if (t < kRgRows) {
+      const int i = wg + t * W;
+      double pv = 0.0;
+      if (t < nrows && i > k) {
+        double sum = 0.0;
+        for (int q = 0; q < kTrdWaves; ++q) sum += rs[t * kTrdWaves + q];
+        const double p = tk * sum;
+        pb[i] = p;
+        pv = p * v[i];
+      }
+      pvs[t] = pv;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double c = 0.0;
+      for (int s = 0; s < kRgRows; ++s) c += pvs[s];
+      a.part[(size_t)par * W + wg] = c;
+    }
+    if (!exchange(a, (unsigned)k + 1, &fail)) return;
+    // gather p, the pivot row and the shares of p.v (one memory round trip)
+    double rp[PER], rx[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = t + kTrdThreads * u;
+      const bool live = j > k && j < P;
+      rp[u] = live ? pb[j] : 0.0;
+      rx[u] = live ? xb[j] : 0.0;
+    }
+    const double share = t < W ? a.part[(size_t)par * W + t] : 0.0;
+    if (k % W == wg) {  // column k of the output
+      for (int j = k + 2 + t; j < P; j += kTrdThreads) a.Vr[(size_t)k * P + j] = v[j];
+      if (t == 0) {
+        a.d[k] = x[k];
+        a.e[k] = beta;
+        a.tau[k] = tk;
+      }
+    }
+    const double dot = block_sum(share, red);
+    const double al = -0.5 * tk * dot;
+    double rw[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = t + kTrdThreads * u;
+      rw[u] = rp[u] + al * rv[u];
+      if (j < P) w[j] = rw[u];
+    }
+    __syncthreads();
+    {
+      const double vk = v[k + 1], wk = w[k + 1];
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int j = t + kTrdThreads * u;
+        if (j > k && j < P) x[j] = rx[u] - (vk * rw[u] + wk * rv[u]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kRgRows; ++s) {
+      const int i = wg + s * W;
+      if (s < nrows && i > k) {
+        const double vi = v[i], wi = w[i];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) A[s][u] -= vi * rw[u] + wi * rv[u];
+      }
+    }
+    __syncthreads();
+  }
+  if ((P - 1) % W == wg && t == 0) a.d[P - 1] = x[P - 1];
+}
+
 // C = Q Z for Q = H_0 H_1 ... H_{P-2} of sytrd_lower (LAPACK dormtr, side
 // left, uplo lower, no transpose).  Each wave owns one column of Z, held in
 // registers (lane l has rows l + 64m, m < M); the reflectors are applied from
@@ -754,7 +934,7 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
 bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr,
                  const int **abort_flag) {
   if (P < 2 || P > kTrdMaxP || P > (u64)kTrdThreads * kTrdMaxPer) return false;
-  if (P <= (u64)kOwMaxP && c.trd_one_wg) {
+  if (P <= (u64)kOwMaxP && c.trd_mode == 0) {
     // one workgroup, no waits between workgroups: the flag stays zero
     c.ws_trd.reserve(sizeof(int));
     int *abort = c.ws_trd.as<int>();
@@ -762,6 +942,36 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
     SP_HIP(hipFuncSetAttribute((const void *)sytrd_ow_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOwLds));
     sytrd_ow_kernel<<<1, kOwThreads, kOwLds, c.stream>>>(OwArgs{G, Vr, (int)P, D, E, tau});
+    SP_CHECK_LAUNCH();
+    *abort_flag = abort;
+    return true;
+  }
+  int ncu = 0;
+  SP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+  const int wmax = std::min(ncu, kTrdThreads);
+  // rows in registers pay off where a workgroup's rows are long (P = 1,716:
+  // 15.5 against 20.6 ms); up to P = 1,024 the LDS form is as fast or faster
+  if ((c.trd_mode == 2 || (c.trd_mode == 0 && P > 2 * (u64)kTrdThreads)) &&
+      P <= 4 * (u64)kTrdThreads && (P + kRgRows - 1) / kRgRows <= (u64)wmax) {
+    // rows in registers: 16 per workgroup at most, one workgroup per CU
+    const int per = P <= (u64)kTrdThreads ? 1 : P <= 2 * (u64)kTrdThreads ? 2 : 4;
+    const void *kern = per == 1   ? (const void *)sytrd_reg_kernel<1>
+                       : per == 2 ? (const void *)sytrd_reg_kernel<2>
+                                  : (const void *)sytrd_reg_kernel<4>;
+    const int W = (int)((P + kRgRows - 1) / kRgRows);
+    const size_t lds = (size_t)(3 * P + kRgRows * kTrdWaves + kRgRows) * sizeof(double);
+    c.ws_trd.reserve((4 * P + 2 * W) * sizeof(double) + 2 * sizeof(unsigned));
+    double *pbuf = c.ws_trd.as<double>();
+    double *xbuf = pbuf + 2 * P;
+    double *part = xbuf + 2 * P;
+    unsigned *ctr = reinterpret_cast<unsigned *>(part + 2 * W);
+    int *abort = reinterpret_cast<int *>(ctr + 1);
+    SP_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), c.stream));
+    SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    TrdArgs a{G, Vr, (int)P, W, kRgRows, D, E, tau, pbuf, xbuf, part, ctr, abort,
+              c.trd_spin_limit};
+    void *args[] = {&a};
+    SP_HIP(hipLaunchKernel(kern, dim3(W), dim3(kTrdThreads), args, lds, c.stream));
     SP_CHECK_LAUNCH();
     *abort_flag = abort;
     return true;
@@ -774,8 +984,6 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
                      : per == 2 ? (const void *)sytrd_lower_kernel<2>
                      : per == 4 ? (const void *)sytrd_lower_kernel<4>
                                 : (const void *)sytrd_lower_kernel<8>;
-  int ncu = 0;
-  SP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
   // the LDS a workgroup may take: 160 KiB less the kernel's static part
   hipFuncAttributes fa;
   SP_HIP(hipFuncGetAttributes(&fa, kern));
@@ -783,7 +991,6 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
   // about SP_SYTRD_ROWS rows per workgroup, more workgroups when the rows do
   // not fit the LDS; one workgroup per CU at most (co-residency) and at most
   // one share of p.v per thread of the final sum
-  const int wmax = std::min(ncu, kTrdThreads);
   int W = (int)std::min<u64>((P + SP_SYTRD_ROWS - 1) / SP_SYTRD_ROWS, (u64)wmax);
   W = std::max(W, 1);
   auto lds_of = [P](int w) {

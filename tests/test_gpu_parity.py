@@ -345,7 +345,8 @@ def spectrum_matrix(P, lam, seed):
     return (Q * lam) @ Q.T
 
 
-@pytest.mark.parametrize("eig", ["syevd", "syevj", "tridiag", "tridiag-wg", "tridiag-rocsolver"])
+@pytest.mark.parametrize("eig", ["syevd", "syevj", "tridiag", "tridiag-wg", "tridiag-reg",
+                                 "tridiag-rocsolver"])
 @pytest.mark.parametrize("P,R,kind", [(1, 1, "random"), (2, 1, "random"), (3, 2, "random"),
                                       (32, 4, "random"), (33, 5, "gram"), (50, 8, "random"),
                                       (97, 8, "random"), (130, 6, "clusters"),
@@ -357,12 +358,13 @@ def test_syevd_top_matches_numpy(P, R, kind, eig, monkeypatch):
     # matrix::syevd: top-R eigenvectors, ascending, against numpy's LAPACK
     if eig == "syevj" and P > 500:
         pytest.skip("Jacobi sweeps at this order take seconds; the order is covered by the others")
-    if eig == "tridiag-wg":
-        # the multi-workgroup reduction also where the one-workgroup form applies
-        if P > 256:
+    if eig in ("tridiag-wg", "tridiag-reg"):
+        # the multi-workgroup reductions (rows in LDS; rows in registers) at
+        # every order, also where the default takes another form
+        if eig == "tridiag-reg" and P > 1024:
             pytest.skip("the same reduction as 'tridiag' at this order")
+        monkeypatch.setenv("SP_SYTRD", eig.split("-")[1])
         eig = "tridiag"
-        monkeypatch.setenv("SP_SYTRD", "wg")
     monkeypatch.setenv("SP_EIG", eig)
     rng = np.random.default_rng(P + R)
     if kind == "random":
