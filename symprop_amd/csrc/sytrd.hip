@@ -654,7 +654,10 @@ __global__ __launch_bounds__(kOwThreads) void sytrd_ow_kernel(OwArgs a) {
 // half-row swaps, DPP mirrors, quad swaps: 16 values in about 60 VALU
 // instructions, lane l ending with row (l/4) mod 16), then over the waves
 // through the LDS in a fixed order.
-constexpr int kRgRows = 16;
+#ifndef SP_SYTRD_REG_ROWS
+#define SP_SYTRD_REG_ROWS 16
+#endif
+constexpr int kRgRows = SP_SYTRD_REG_ROWS;  // rows per workgroup at most (a multiple of 16)
 
 // (a' + b') of v_permlane32_swap(a, b) / v_permlane16_swap(a, b): the lanes
 // of the lower half-wave (even half-rows) get their a plus the upper (odd)
@@ -694,7 +697,7 @@ __device__ inline double rg_reduce16(const double x[16], int lane) {
   return u;
 }
 
-template <int PER>
+template <int PER, int ROWS>
 __global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
   extern __shared__ double lds[];
   __shared__ double red[kTrdWaves];
@@ -704,12 +707,12 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
   double *v = lds;                    // [P] reflector of the column
   double *w = v + P;                  // [P]
   double *x = w + P;                  // [P] pivot row
-  double *rs = x + P;                 // [16][waves] each wave's sums of the local rows
-  double *pvs = rs + kRgRows * kTrdWaves;  // [16] p_i v_i of the local rows
+  double *rs = x + P;                 // [ROWS][waves] each wave's sums of the local rows
+  double *pvs = rs + ROWS * kTrdWaves;  // [ROWS] p_i v_i of the local rows
   const int nrows = wg < P ? (P - wg + W - 1) / W : 0;
-  double A[kRgRows][PER];
+  double A[ROWS][PER];
 #pragma unroll
-  for (int s = 0; s < kRgRows; ++s)
+  for (int s = 0; s < ROWS; ++s)
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int j = t + kTrdThreads * u;
@@ -728,23 +731,24 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
       const int j = t + kTrdThreads * u;
       rv[u] = j > k && j < P ? v[j] : 0.0;
     }
-    {
-      double ps[kRgRows];
 #pragma unroll
-      for (int s = 0; s < kRgRows; ++s) {
-        double acc = A[s][0] * rv[0];
+    for (int g = 0; g < ROWS / 16; ++g) {
+      double ps[16];
 #pragma unroll
-        for (int u = 1; u < PER; ++u) acc += A[s][u] * rv[u];
+      for (int s = 0; s < 16; ++s) {
+        double acc = A[16 * g + s][0] * rv[0];
+#pragma unroll
+        for (int u = 1; u < PER; ++u) acc += A[16 * g + s][u] * rv[u];
         ps[s] = acc;
       }
       const double r = rg_reduce16(ps, lane);
-      if ((lane & 3) == 0) rs[((lane >> 2) & 15) * kTrdWaves + wave] = r;
+      if ((lane & 3) == 0) rs[(16 * g + ((lane >> 2) & 15)) * kTrdWaves + wave] = r;
     }
     double *xb = a.xbuf + (size_t)par * P;
     if ((k + 1) % W == wg) {  // row k+1 as it stands, for every workgroup
       const int s1 = (k + 1) / W;
 #pragma unroll
-      for (int s = 0; s < kRgRows; ++s)
+      for (int s = 0; s < ROWS; ++s)
         if (s == s1) {
 #pragma unroll
           for (int u = 0; u < PER; ++u) {
@@ -755,7 +759,7 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
     }
     __syncthreads();
     double *pb = a.pbuf + (size_t)par * P;
-    if (t < kRgRows) {
+    if (t < ROWS) {
       const int i = wg + t * W;
       double pv = 0.0;
       if (t < nrows && i > k) {
@@ -770,7 +774,7 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
     __syncthreads();
     if (t == 0) {
       double c = 0.0;
-      for (int s = 0; s < kRgRows; ++s) c += pvs[s];
+      for (int s = 0; s < ROWS; ++s) c += pvs[s];
       a.part[(size_t)par * W + wg] = c;
     }
     if (!exchange(a, (unsigned)k + 1, &fail)) return;
@@ -811,7 +815,7 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_reg_kernel(TrdArgs a) {
       }
     }
 #pragma unroll
-    for (int s = 0; s < kRgRows; ++s) {
+    for (int s = 0; s < ROWS; ++s) {
       const int i = wg + s * W;
       if (s < nrows && i > k) {
         const double vi = v[i], wi = w[i];
@@ -955,9 +959,9 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
       P <= 4 * (u64)kTrdThreads && (P + kRgRows - 1) / kRgRows <= (u64)wmax) {
     // rows in registers: 16 per workgroup at most, one workgroup per CU
     const int per = P <= (u64)kTrdThreads ? 1 : P <= 2 * (u64)kTrdThreads ? 2 : 4;
-    const void *kern = per == 1   ? (const void *)sytrd_reg_kernel<1>
-                       : per == 2 ? (const void *)sytrd_reg_kernel<2>
-                                  : (const void *)sytrd_reg_kernel<4>;
+    const void *kern = per == 1   ? (const void *)sytrd_reg_kernel<1, kRgRows>
+                       : per == 2 ? (const void *)sytrd_reg_kernel<2, kRgRows>
+                                  : (const void *)sytrd_reg_kernel<4, kRgRows>;
     const int W = (int)((P + kRgRows - 1) / kRgRows);
     const size_t lds = (size_t)(3 * P + kRgRows * kTrdWaves + kRgRows) * sizeof(double);
     c.ws_trd.reserve((4 * P + 2 * W) * sizeof(double) + 2 * sizeof(unsigned));
