@@ -21,7 +21,6 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <limits>
 
@@ -1049,9 +1048,13 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
   // eigen-step below, every path of which synchronises the stream
   scale_to_unit_async(c, c.d_G.as<double>(), P * P);
   bool warm = false;
-  auto ms_since = [](std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
-        .count();
+  // the forms are timed between events on the stream: the time from when the
+  // stream reaches the step (after the queued S3TTMc and Gram) to its end
+  hipEvent_t *ev = c.eig_ev.get();
+  auto ms_between = [](hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    SP_HIP(hipEventElapsedTime(&ms, a, b));
+    return (double)ms;
   };
   auto mean = [](double m, double x) { return m > 0.0 ? 0.75 * m + 0.25 * x : x; };
   if (c.warm_enabled && c.eig_method == 2) {
@@ -1059,7 +1062,7 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
       --c.warm_skip;
     } else {
       double res;
-      const auto t0 = std::chrono::steady_clock::now();
+      SP_HIP(hipEventRecord(ev[0], c.stream));
       // P <= 256: the block Krylov form; above, the one- or two-block step.
       // Its number of blocks m adapts: two more after a miss, and that m+1 is
       // then the floor for the next 64 calls; one fewer after a residual two
@@ -1074,6 +1077,7 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
         res = kry ? eig_top_krylov(c, c.d_G.as<double>(), P, R, c.kry_m, V.as<double>(), theta)
                   : eig_top_warm(c, c.d_G.as<double>(), P, R, V.as<double>(), theta);
       }
+      SP_HIP(hipEventRecord(ev[1], c.stream));
       if (kry) {
         if (res > 1e-13) {
           c.kry_floor = std::min(mmax, c.kry_m + 1);
@@ -1094,7 +1098,8 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
       if (warm) {
         ScopedTimer mark(c, "eig_warm_ok");  // counted success
       } else {
-        c.warm_miss_ms = mean(c.warm_miss_ms, ms_since(t0));
+        SP_HIP(hipEventSynchronize(ev[1]));
+        c.warm_miss_ms = mean(c.warm_miss_ms, ms_between(ev[0], ev[1]));
         if (res > 1e-6 || widest)
           c.warm_skip = c.direct_eig_ms > 0.0
                             ? (u32)std::min(64.0, std::ceil(c.warm_miss_ms /
@@ -1104,10 +1109,11 @@ void dense_leftsvd_y(Ctx &c, double *sigma) {
     }
   }
   if (!warm) {
-    const auto t0 = std::chrono::steady_clock::now();
+    SP_HIP(hipEventRecord(ev[2], c.stream));
     eig_top(c, c.d_G.as<double>(), P, R, V.as<double>(), theta, true);
-    SP_HIP(hipStreamSynchronize(c.stream));
-    c.direct_eig_ms = mean(c.direct_eig_ms, ms_since(t0));
+    SP_HIP(hipEventRecord(ev[3], c.stream));
+    SP_HIP(hipEventSynchronize(ev[3]));
+    c.direct_eig_ms = mean(c.direct_eig_ms, ms_between(ev[2], ev[3]));
   }
   const int ex = c.h_pin.get()[0];
   for (u32 r = 0; r < R; ++r) theta[r] = std::ldexp(theta[r], -ex);

@@ -143,6 +143,24 @@ struct PinWords {
   }
 };
 
+// Four timing events, created on first use (hipEventDefault, so elapsed times
+// can be read) and destroyed with the owner.
+struct EventSet {
+  hipEvent_t e[4] = {};
+  EventSet() = default;
+  EventSet(const EventSet &) = delete;
+  EventSet &operator=(const EventSet &) = delete;
+  ~EventSet() {
+    for (hipEvent_t &x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  hipEvent_t *get() {
+    for (hipEvent_t &x : e)
+      if (!x) SP_HIP(hipEventCreate(&x));
+    return e;
+  }
+};
+
 struct Ctx {
   u32 order = 0, dim = 0;
   u64 nnz = 0;
@@ -221,13 +239,15 @@ struct Ctx {
   // leftsvd's warm-started Rayleigh-Ritz eigen-step (dense.hip): on by default
   // (SP_EIG_WARM=0 at context creation turns it off).  After a miss the next
   // warm_skip calls go straight to the direct solver, sized from the measured
-  // times (running means, ms) so that misses cost at most a twentieth of the
-  // direct path's time; a near miss or a success clears it
+  // times (running means, ms, taken with eig_ev on the stream so that work
+  // queued before the step is not counted) so that misses cost at most a
+  // twentieth of the direct path's time; a near miss or a success clears it
   bool warm_enabled = true;
   u32 warm_skip = 0;
   u32 kry_m = 0;  // blocks of the Krylov form for P <= 256 (0: not chosen yet)
   u32 kry_floor = 2, kry_hold = 0;  // least m while kry_hold calls remain
   double warm_miss_ms = 0.0, direct_eig_ms = 0.0;
+  EventSet eig_ev;
   // host staging of the dense Y-side steps, kept across calls so that a
   // host-to-device copy from them needs no wait before the function returns
   // (the next write to one follows a synchronisation of the stream)
