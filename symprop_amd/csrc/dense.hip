@@ -84,13 +84,29 @@ atb_partial(const double *__restrict__ A, u32 R, const double *__restrict__ B,
   }
 }
 
+// (((0 + x_0) + x_1) + ...) + x_{count-1} with x_i = p[i * stride]: the same
+// additions in the same order as the plain loop (so the same bits), with the
+// loads of eight terms issued before their additions rather than one load
+// wait per term.
+__device__ inline double sum_strided(const double *__restrict__ p, u64 stride, u64 count) {
+  double s = 0.0;
+  u64 i = 0;
+  for (; i + 8 <= count; i += 8) {
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = p[(i + k) * stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += x[k];
+  }
+  for (; i < count; ++i) s += p[i * stride];
+  return s;
+}
+
 __global__ void sum_partials(const double *__restrict__ partial, u64 nb, u64 len,
                              double *__restrict__ out) {
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= len) return;
-  double s = 0.0;
-  for (u64 b = 0; b < nb; ++b) s += partial[b * len + t];
-  out[t] = s;
+  out[t] = sum_strided(partial + t, len, nb);
 }
 
 // The same sums for mid-size outputs: a block takes 64 consecutive entries
@@ -103,8 +119,7 @@ __global__ void __launch_bounds__(256) sum_partials_cols(const double *__restric
   const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
   const u64 t = (u64)blockIdx.x * 64 + e;
   double s = 0.0;
-  if (t < len)
-    for (u64 b = q; b < nb; b += 4) s += partial[b * len + t];
+  if (t < len && (u64)q < nb) s = sum_strided(partial + (u64)q * len + t, 4 * len, (nb - q + 3) / 4);
   acc[q][e] = s;
   __syncthreads();
   if (q == 0 && t < len) out[t] = (acc[0][e] + acc[1][e]) + (acc[2][e] + acc[3][e]);
@@ -305,8 +320,7 @@ __global__ void gram_finish(const double *__restrict__ partial, u64 nk, u64 ntil
   const u64 p = (u64)tl.x * GT + u, q = (u64)tl.y * GT + v;
   if (p >= P || q >= P) return;
   if (tl.x == tl.y && u > v) return;  // each symmetric pair written once
-  double s = 0.0;
-  for (u64 k = 0; k < nk; ++k) s += partial[(k * ntiles + tile) * GT * GT + u * GT + v];
+  double s = sum_strided(partial + tile * GT * GT + u * GT + v, ntiles * GT * GT, nk);
   s *= sqrt(W[p]) * sqrt(W[q]);
   G[p * P + q] = s;
   G[q * P + p] = s;
@@ -506,6 +520,121 @@ void atb(Ctx &c, const double *A, u32 R, const double *B, u64 P, u64 n, double *
   }
 }
 
+// The R x R step of orthonormalize_gram's symmetric pass on the device
+// (R <= kOrthMaxR, one wavefront), so that it needs no host round trip:
+// cyclic Jacobi on (G + G^T)/2 with the round-robin ordering, R'/2 disjoint
+// rotations per round (R' = R rounded up to even; an odd R's extra index
+// takes none) and the host solver's stopping rule (off-diagonal sum of
+// squares at most 1e-30 of the total, at most 100 sweeps); then the
+// eigenvalues in descending order, the count of those above kGramRankTol of
+// the largest, and M = sum over those of v v^T / sqrt(w) written over G.  On
+// the nearly orthonormal factor this pass sees, G is within rounding of the
+// identity and one or two sweeps suffice.  A round writes A' = J^T A J and
+// V' = V J element by element into the other of two buffers: with
+// row i' = alpha_i row i + beta_i row pi_i (pi_i: i's partner),
+// A'[i][j] = alpha_i (alpha_j A[i][j] + beta_j A[i][pi_j])
+//          + beta_i (alpha_j A[pi_i][j] + beta_j A[pi_i][pi_j]).
+constexpr int kOrthMaxR = 16;
+
+__global__ void __launch_bounds__(64) orth_small_sym(double *__restrict__ g, int R) {
+  constexpr int S = kOrthMaxR + 1;  // row stride
+  __shared__ double A[2][kOrthMaxR * S], V[2][kOrthMaxR * S];
+  __shared__ double al[kOrthMaxR], be[kOrthMaxR], w[kOrthMaxR];
+  __shared__ int pi[kOrthMaxR], ord[kOrthMaxR], ngood;
+  const int l = threadIdx.x;
+  for (int e = l; e < kOrthMaxR * kOrthMaxR; e += 64) {
+    const int i = e / kOrthMaxR, j = e % kOrthMaxR;
+    A[0][i * S + j] = i < R && j < R ? 0.5 * (g[i * R + j] + g[j * R + i]) : 0.0;
+    V[0][i * S + j] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int Rp = R + (R & 1), half = Rp / 2;
+  int cur = 0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int e = l; e < R * R; e += 64) {
+      const int i = e / R, j = e % R;
+      const double a = A[cur][i * S + j];
+      tot += a * a;
+      if (i != j) off += a * a;
+    }
+    off = wave_sum(off);
+    tot = wave_sum(tot);
+    if (off <= 1e-30 * tot || off == 0.0) break;  // the same in every lane
+    for (int rnd = 0; rnd < Rp - 1; ++rnd) {
+      const double *Ac = A[cur], *Vc = V[cur];
+      if (l < half) {
+        // pairs of the round: (rnd, R'-1) and (rnd + l, rnd - l) mod R'-1
+        int p = l == 0 ? rnd : (rnd + l) % (Rp - 1);
+        int q = l == 0 ? Rp - 1 : (rnd - l + Rp - 1) % (Rp - 1);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1.0, sn = 0.0;
+        if (q < R && Ac[p * S + q] != 0.0) {
+          // t = tan of the angle that zeroes A[p][q]: the host's
+          // sgn(theta) / (|theta| + sqrt(theta^2 + 1)), theta = tau / sigma,
+          // with |sigma| multiplied into numerator and denominator
+          const double tau = Ac[q * S + q] - Ac[p * S + p], sigma = 2.0 * Ac[p * S + q];
+          const double t = (tau >= 0.0 ? sigma : -sigma) / (fabs(tau) + sqrt(tau * tau + sigma * sigma));
+          c = 1.0 / sqrt(t * t + 1.0);
+          sn = t * c;
+        }
+        al[p] = c;
+        be[p] = -sn;
+        pi[p] = q < R ? q : p;
+        if (q < R) {
+          al[q] = c;
+          be[q] = sn;
+          pi[q] = p;
+        }
+      }
+      __syncthreads();
+      double *An = A[cur ^ 1], *Vn = V[cur ^ 1];
+      for (int e = l; e < R * R; e += 64) {
+        const int i = e / R, j = e % R;
+        const int pi_i = pi[i], pi_j = pi[j];
+        const double ai = al[i], bi = be[i], aj = al[j], bj = be[j];
+        const double x = aj * Ac[i * S + j] + bj * Ac[i * S + pi_j];
+        const double y = aj * Ac[pi_i * S + j] + bj * Ac[pi_i * S + pi_j];
+        An[i * S + j] = ai * x + bi * y;
+        Vn[i * S + j] = aj * Vc[i * S + j] + bj * Vc[i * S + pi_j];
+      }
+      cur ^= 1;
+      __syncthreads();
+    }
+  }
+  const double *Ac = A[cur], *Vc = V[cur];
+  // descending order; a strict total order (NaN lowest, ties by index), so
+  // the ranks are a permutation
+  if (l < R) {
+    const double a = Ac[l * S + l], ka = a == a ? a : -HUGE_VAL;
+    int rank = 0;
+    for (int j = 0; j < R; ++j) {
+      const double b = Ac[j * S + j], kb = b == b ? b : -HUGE_VAL;
+      rank += kb > ka || (kb == ka && j < l);
+    }
+    ord[rank] = l;
+    w[rank] = a;
+  }
+  __syncthreads();
+  if (l == 0) {
+    int n = 0;
+    while (n < R && isfinite(w[n]) && w[n] > 0.0 && w[n] > w[0] * kGramRankTol) ++n;
+    ngood = n;
+  }
+  __syncthreads();
+  const int ng = ngood;
+  for (int e = l; e < R * R; e += 64) {  // M[r][s], r = output column
+    const int r = e / R, s = e % R;
+    double v = 0.0;
+    for (int q = 0; q < ng; ++q) v += Vc[s * S + ord[q]] * Vc[r * S + ord[q]] / sqrt(w[q]);
+    g[e] = v;
+  }
+}
+
 // U <- U V Lambda^{-1/2} from the R x R Gram U^T U: the left singular
 // vectors of U in descending order (first pass), or U <- U (U^T U)^{-1/2}
 // (symmetric pass, which keeps the column order of a nearly orthonormal U).
@@ -551,6 +680,24 @@ u32 orthonormalize_gram(Ctx &c, DevBuf &tmp, bool symmetric) {
   SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
                         c.stream));
   return good;
+}
+
+// orthonormalize_gram's symmetric pass with its R x R step on the device
+// (orth_small_sym, R <= kOrthMaxR): nothing waits.
+void orthonormalize_sym_dev(Ctx &c, DevBuf &tmp) {
+  const u32 R = c.R;
+  const u64 n = c.dim;
+  DevBuf &g = c.ws_rr;
+  g.reserve((u64)R * R * sizeof(double));
+  atb(c, c.d_U.as<double>(), R, c.d_U.as<double>(), R, n, g.as<double>(), "gram_u");
+  orth_small_sym<<<1, 64, 0, c.stream>>>(g.as<double>(), (int)R);
+  SP_CHECK_LAUNCH();
+  tmp.reserve(n * R * sizeof(double));
+  y_times_mt<<<ymt_grid(n, R), 256, 0, c.stream>>>(c.d_U.as<double>(), n, R,
+                                                   g.as<double>(), R, tmp.as<double>());
+  SP_CHECK_LAUNCH();
+  SP_HIP(hipMemcpyAsync(c.d_U.p, tmp.p, n * R * sizeof(double), hipMemcpyDeviceToDevice,
+                        c.stream));
 }
 
 // Host: L^{-1} (row-major, lower) of the Cholesky factor of the R x R Gram G
@@ -1200,8 +1347,13 @@ void dense_regularize(Ctx &c, double delta) {
   const u64 n = c.dim;
   clip_rows<<<grid_for(n, 256), 256, 0, c.stream>>>(c.d_U.as<double>(), n, c.R, delta);
   SP_CHECK_LAUNCH();
+  // first pass on the host: its count of kept directions decides the
+  // completion; the second pass sees orthonormal columns
   complete_basis(c, orthonormalize_gram(c, c.ws_nr, false));
-  orthonormalize_gram(c, c.ws_nr, true);
+  if (c.R <= (u32)kOrthMaxR)
+    orthonormalize_sym_dev(c, c.ws_nr);
+  else
+    orthonormalize_gram(c, c.ws_nr, true);
 }
 
 void dense_factor_snapshot(Ctx &c) {

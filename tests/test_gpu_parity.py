@@ -554,6 +554,33 @@ def test_regularize_matches_reference_math():
     assert np.allclose(np.abs(np.sum(Un * Us, axis=0)), 1.0, atol=1e-8)
 
 
+@pytest.mark.parametrize("R", [3, 10, 16, 17])
+def test_regularize_of_orthonormal_factor(R):
+    # Repeated calls, as in the Tucker loops: for R <= 16 the symmetric second
+    # pass runs its R x R step on the device (one-wave Jacobi), above that on
+    # the host; both must give the clipped factor's left singular vectors
+    rng = np.random.default_rng(40 + R)
+    n, delta = 300, 0.2
+    t = sp.SpSymtensor(n, 3, np.array([[1, 2, 3], [2, 3, 4]], np.uint32), np.ones(2))
+    t.set_factor(rng.standard_normal((n, R)))
+    say("regularize twice", R)
+    t.regularize(delta)
+    U1 = t.factor()
+    for _ in range(2):
+        t.regularize(delta)
+        Un = t.factor()
+        nr = np.linalg.norm(U1, axis=1, keepdims=True)
+        Uc = np.where(nr > delta, U1 * (delta / nr), U1)
+        Us, sv, _ = np.linalg.svd(Uc, full_matrices=False)
+        assert np.allclose(Un.T @ Un, np.eye(R), atol=1e-12)
+        assert np.allclose(np.linalg.svd(Un.T @ Us, compute_uv=False), 1.0, atol=1e-10)
+        # singular vectors up to sign where the singular values are apart
+        gap = np.minimum(np.r_[np.inf, sv[:-1] - sv[1:]], np.r_[sv[:-1] - sv[1:], np.inf])
+        sep = gap > 1e-6 * sv[0]
+        assert np.allclose(np.abs(np.sum(Un * Us, axis=0))[sep], 1.0, atol=1e-8)
+        U1 = Un
+
+
 # ---- HOEVD initialisation (spsymtensor.cpp:363-439) --------------------------
 
 @pytest.mark.parametrize("N,n,cnt,seed", [(3, 7, 40, 1), (4, 9, 120, 2), (5, 6, 80, 3),
