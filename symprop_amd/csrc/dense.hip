@@ -242,19 +242,30 @@ __global__ void scale_cols(const double *__restrict__ core, const double *__rest
 constexpr int GT = 64;
 constexpr int GK = 32;
 constexpr int GL = GK * GT / 256;  // slab elements per thread
-__global__ void __launch_bounds__(256)
-gram_partial(const double *__restrict__ Y, u64 n, u64 P, u64 rows_per,
+__global__ void __launch_bounds__(256, 4)
+gram_partial(const double *__restrict__ Y, u64 n, u64 P, u64 rows_per, u32 ntiles, u32 nk,
              const int2 *__restrict__ tiles, double *__restrict__ partial) {
   // FP64 MFMA: wavefront w computes the 32 x 32 quadrant (w/2, w%2) of the
   // 64 x 64 tile as 2 x 2 blocks of v_mfma_f64_16x16x4f64 with K = rows.
   // The next GK-row slab is loaded into registers while the current one in
   // LDS is multiplied.
+  //
+  // Workgroups are dealt to the 8 XCDs round-robin by linear id, so block b
+  // runs on XCD b % 8.  The work items (chunk, tile), chunk-major, are split
+  // into 8 equal contiguous ranges and block 8 s + x takes item s of range x:
+  // each XCD gets the same number of items, and the tiles of a row chunk run
+  // on one XCD at about the same time, reading the chunk's rows of Y from
+  // that XCD's L2 after the first reader.  Blocks past the end leave at once.
   __shared__ double sa[GK][GT + 1];
   __shared__ double sb[GK][GT + 1];
-  const int2 tl = tiles[blockIdx.x];
+  const u32 items = ntiles * nk, per = (items + 7) / 8;
+  const u32 slot = blockIdx.x >> 3, item = (blockIdx.x & 7) * per + slot;
+  if (slot >= per || item >= items) return;  // the whole block, before any barrier
+  const u32 tile = item % ntiles, chunk = item / ntiles;
+  const int2 tl = tiles[tile];
   const u64 p0 = (u64)tl.x * GT, q0 = (u64)tl.y * GT;
   const bool diag = tl.x == tl.y;
-  const u64 i0 = (u64)blockIdx.y * rows_per;
+  const u64 i0 = (u64)chunk * rows_per;
   const u64 i1 = min(n, i0 + rows_per);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kr = lane >> 4, kc = lane & 15;
@@ -299,7 +310,7 @@ gram_partial(const double *__restrict__ Y, u64 n, u64 P, u64 rows_per,
     }
     __syncthreads();
   }
-  double *out = partial + ((u64)blockIdx.y * gridDim.x + blockIdx.x) * GT * GT;
+  double *out = partial + ((u64)chunk * ntiles + tile) * GT * GT;
 #pragma unroll
   for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -828,8 +839,9 @@ void dense_gram(Ctx &c) {
   for (u64 a = 0; a < nt; ++a)
     for (u64 b = a; b < nt; ++b) tiles.push_back(make_int2((int)a, (int)b));
   const u64 ntiles = tiles.size();
-  // about 2048 blocks in all, at least 128 rows each, rows a multiple of GK
-  u64 nk = std::max<u64>(1, std::min<u64>((n + 127) / 128, 2048 / ntiles + 1));
+  // at most 2048 blocks (two rounds at four per CU), at least 128 rows
+  // each, rows a multiple of GK
+  u64 nk = std::max<u64>(1, std::min<u64>((n + 127) / 128, 2048 / ntiles));
   const u64 rows_per = ((n + nk - 1) / nk + GK - 1) / GK * GK;
   nk = (n + rows_per - 1) / rows_per;
   DevBuf &dt = c.ws_tiles;
@@ -844,8 +856,10 @@ void dense_gram(Ctx &c) {
   c.d_G.reserve(P * P * sizeof(double));
   {
     ScopedTimer t(c, "gram");
-    gram_partial<<<dim3((unsigned)ntiles, (unsigned)nk), 256, 0, c.stream>>>(
-        c.d_Y.as<double>(), n, P, rows_per, dt.as<int2>(), c.d_scratch.as<double>());
+    // 8 ranges of ceil(ntiles nk / 8) items: see gram_partial's XCD mapping
+    gram_partial<<<(unsigned)(8 * ((ntiles * nk + 7) / 8)), 256, 0, c.stream>>>(
+        c.d_Y.as<double>(), n, P, rows_per, (u32)ntiles, (u32)nk, dt.as<int2>(),
+        c.d_scratch.as<double>());
     SP_CHECK_LAUNCH();
     gram_finish<<<grid_for(ntiles * GT * GT, 256), 256, 0, c.stream>>>(
         c.d_scratch.as<double>(), nk, ntiles, dt.as<int2>(), P, c.d_W.as<double>(),
