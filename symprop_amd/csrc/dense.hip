@@ -200,18 +200,38 @@ y_times_mt(const double *__restrict__ Y, u64 n, u64 P, const double *__restrict_
   const u64 row0 = (u64)blockIdx.x * 64;
   const u32 r0 = blockIdx.y * 16;
   d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  for (u64 k0 = 0; k0 < P; k0 += YK) {
-    for (int t = threadIdx.x; t < 64 * YK; t += 256) {
-      const int rr = t / YK, kk = t % YK;
+  // the next slab is loaded into registers while the current one in LDS is
+  // multiplied (thread t holds elements t + 256 j of each slab)
+  constexpr int NY = 64 * YK / 256, NM = 16 * YK / 256;
+  double ry[NY], rm[NM];
+  auto fetch = [&](u64 k0) {
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      const int t = threadIdx.x + 256 * j, rr = t / YK, kk = t % YK;
       const u64 i = row0 + rr, k = k0 + kk;
-      sy[rr][kk] = (i < n && k < P) ? Y[i * P + k] : 0.0;
+      ry[j] = (i < n && k < P) ? Y[i * P + k] : 0.0;
     }
-    for (int t = threadIdx.x; t < 16 * YK; t += 256) {
-      const int rr = t / YK, kk = t % YK;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      const int t = threadIdx.x + 256 * j, rr = t / YK, kk = t % YK;
       const u64 k = k0 + kk;
-      sm[rr][kk] = (r0 + rr < R && k < P) ? M[(u64)(r0 + rr) * P + k] : 0.0;
+      rm[j] = (r0 + rr < R && k < P) ? M[(u64)(r0 + rr) * P + k] : 0.0;
+    }
+  };
+  fetch(0);
+  for (u64 k0 = 0; k0 < P; k0 += YK) {
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      const int t = threadIdx.x + 256 * j;
+      sy[t / YK][t % YK] = ry[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      const int t = threadIdx.x + 256 * j;
+      sm[t / YK][t % YK] = rm[j];
     }
     __syncthreads();
+    if (k0 + YK < P) fetch(k0 + YK);
 #pragma unroll
     for (int ks = 0; ks < YK / 4; ks += 2) {
       // A[m][k] = Y[row0 + 16 wave + m][k0 + 4 ks + k], B[k][nn] = M[r0 + nn][k0 + 4 ks + k]
