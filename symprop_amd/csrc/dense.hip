@@ -926,8 +926,10 @@ void eig_top_tridiag(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<dou
       // the reduction's output and its time-out flag in one wait
       SP_HIP(hipMemcpyAsync(hd.data(), D, 2 * P * sizeof(double), hipMemcpyDeviceToHost,
                             c.stream));
-      SP_HIP(hipMemcpyAsync(&c.h_trd_abort, abort_flag, sizeof(int), hipMemcpyDeviceToHost,
-                            c.stream));
+      c.h_trd_abort = 0;
+      if (abort_flag)
+        SP_HIP(hipMemcpyAsync(&c.h_trd_abort, abort_flag, sizeof(int), hipMemcpyDeviceToHost,
+                              c.stream));
       SP_HIP(hipStreamSynchronize(c.stream));
       if (c.h_trd_abort) {
         // some workgroup was not resident in time (the device is shared with

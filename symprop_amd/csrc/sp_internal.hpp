@@ -368,7 +368,7 @@ void eig_top(Ctx &c, double *G, u64 P, u32 R, double *V, std::vector<double> &th
 // fit the kernel's LDS budget; otherwise *abort_flag points to a device word
 // that is nonzero once the kernel has run if a wait between its workgroups
 // timed out (then every workgroup left, G is untouched and D, E, tau, Vr are
-// not valid).
+// not valid), or is null for the one-workgroup form, which has no such wait.
 bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *tau, double *Vr,
                  const int **abort_flag);
 constexpr u64 kTrdMaxP = 2048;  // sytrd_lower returns false beyond this order

@@ -939,15 +939,13 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
                  const int **abort_flag) {
   if (P < 2 || P > kTrdMaxP || P > (u64)kTrdThreads * kTrdMaxPer) return false;
   if (P <= (u64)kOwMaxP && c.trd_mode == 0) {
-    // one workgroup, no waits between workgroups: the flag stays zero
-    c.ws_trd.reserve(sizeof(int));
-    int *abort = c.ws_trd.as<int>();
-    SP_HIP(hipMemsetAsync(abort, 0, sizeof(int), c.stream));
+    // one workgroup, no waits between workgroups: nothing can time out, so
+    // there is no flag to clear or read
     SP_HIP(hipFuncSetAttribute((const void *)sytrd_ow_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOwLds));
     sytrd_ow_kernel<<<1, kOwThreads, kOwLds, c.stream>>>(OwArgs{G, Vr, (int)P, D, E, tau});
     SP_CHECK_LAUNCH();
-    *abort_flag = abort;
+    *abort_flag = nullptr;
     return true;
   }
   int ncu = 0;
