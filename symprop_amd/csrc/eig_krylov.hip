@@ -389,8 +389,7 @@ bool krylov_basis(Ctx &c, const double *G, const double *X0t, u64 P, u32 b, u32 
   // block widths 4, 8, 12, 16: the factorisation of each block is quadratic
   // in the width, padded rows included
   auto launch = [&](auto kern, size_t lds) {
-    SP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
+    allow_dyn_lds(c.device, (const void *)kern, lds);
     kern<<<1, kKyThreads, lds, c.stream>>>(a);
   };
   if (b <= 4) launch(krylov_basis_kernel<4>, ky_lds<4>());

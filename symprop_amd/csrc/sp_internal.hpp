@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -424,6 +426,32 @@ struct ScopedTimer {
     c.timers.push_back(t);
   }
 };
+
+// Kernel attributes that the launchers need, set or queried once per
+// (device, kernel) per process instead of at every launch: the dynamic-LDS
+// limit is raised only when a launch asks for more than was allowed so far,
+// and the static LDS size is read once.
+inline void allow_dyn_lds(int device, const void *kern, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, size_t> allowed;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t &cur = allowed[{device, kern}];
+  if (bytes <= cur) return;
+  SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  cur = bytes;
+}
+
+inline size_t static_lds(int device, const void *kern) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, size_t> known;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = known.find({device, kern});
+  if (it != known.end()) return it->second;
+  hipFuncAttributes fa;
+  SP_HIP(hipFuncGetAttributes(&fa, kern));
+  known[{device, kern}] = fa.sharedSizeBytes;
+  return fa.sharedSizeBytes;
+}
 
 inline unsigned grid_for(u64 n, unsigned block) {
   u64 g = (n + block - 1) / block;

@@ -919,14 +919,12 @@ bool apply_q_lower(Ctx &c, const double *G, const double *tau, u64 P, const doub
                      : M == 8  ? (const void *)apply_q_lower_kernel<8>
                      : M == 16 ? (const void *)apply_q_lower_kernel<16>
                                : (const void *)apply_q_lower_kernel<32>;
-  hipFuncAttributes fa;
-  SP_HIP(hipFuncGetAttributes(&fa, kern));
   // reflectors per LDS stage: as many as fit, at most 32 (one tau each)
-  const size_t budget = kTrdLds - fa.sharedSizeBytes;
+  const size_t budget = kTrdLds - static_lds(c.device, kern);
   const int B = (int)std::min<u64>(32, budget / ((P + 1) * sizeof(double)));
   if (B < 1) return false;
   const size_t lds = (size_t)B * (P + 1) * sizeof(double);
-  SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  allow_dyn_lds(c.device, kern, lds);
   int Pi = (int)P, Ri = (int)R;
   void *args[] = {(void *)&G, (void *)&tau, &Pi, (void *)&B, (void *)&Z, &Ri, (void *)&V};
   SP_HIP(hipLaunchKernel(kern, dim3((R + kAqWaves - 1) / kAqWaves), dim3(kAqThreads), args, lds,
@@ -941,8 +939,7 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
   if (P <= (u64)kOwMaxP && c.trd_mode == 0) {
     // one workgroup, no waits between workgroups: nothing can time out, so
     // there is no flag to clear or read
-    SP_HIP(hipFuncSetAttribute((const void *)sytrd_ow_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kOwLds));
+    allow_dyn_lds(c.device, (const void *)sytrd_ow_kernel, kOwLds);
     sytrd_ow_kernel<<<1, kOwThreads, kOwLds, c.stream>>>(OwArgs{G, Vr, (int)P, D, E, tau});
     SP_CHECK_LAUNCH();
     *abort_flag = nullptr;
@@ -969,7 +966,7 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
     unsigned *ctr = reinterpret_cast<unsigned *>(part + 2 * W);
     int *abort = reinterpret_cast<int *>(ctr + 1);
     SP_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), c.stream));
-    SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    allow_dyn_lds(c.device, kern, lds);
     TrdArgs a{G, Vr, (int)P, W, kRgRows, D, E, tau, pbuf, xbuf, part, ctr, abort,
               c.trd_spin_limit};
     void *args[] = {&a};
@@ -987,9 +984,7 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
                      : per == 4 ? (const void *)sytrd_lower_kernel<4>
                                 : (const void *)sytrd_lower_kernel<8>;
   // the LDS a workgroup may take: 160 KiB less the kernel's static part
-  hipFuncAttributes fa;
-  SP_HIP(hipFuncGetAttributes(&fa, kern));
-  const size_t budget = kTrdLds - fa.sharedSizeBytes;
+  const size_t budget = kTrdLds - static_lds(c.device, kern);
   // about SP_SYTRD_ROWS rows per workgroup, more workgroups when the rows do
   // not fit the LDS; one workgroup per CU at most (co-residency) and at most
   // one share of p.v per thread of the final sum
@@ -1011,7 +1006,7 @@ bool sytrd_lower(Ctx &c, const double *G, u64 P, double *D, double *E, double *t
   unsigned *ctr = reinterpret_cast<unsigned *>(part + 2 * W);
   int *abort = reinterpret_cast<int *>(ctr + 1);
   SP_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), c.stream));
-  SP_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  allow_dyn_lds(c.device, kern, lds);
   TrdArgs a{G, Vr, (int)P, W, nloc, D, E, tau, pbuf, xbuf, part, ctr, abort, c.trd_spin_limit};
   // An ordinary launch: W <= CUs workgroups, each within one CU's LDS, are
   // resident together on the device.  (A cooperative launch would also
